@@ -1,0 +1,225 @@
+// bd_dev.h -- device-side data layout and inline physics primitives.
+//
+// Layout in HBM (one context = one fiber):
+//   * bead state in the reference AoS layout x[3i+k] (r, a, b, c, r_t, r_lh),
+//     so host buffers of the reference driver copy 1:1;
+//   * core charges and globular-head (LHG) sites are expanded to world
+//     coordinates once per force pass (SoA-free AoS double triples);
+//   * per-bead outputs are written by exactly one thread (gather form), so
+//     every force is a deterministic fixed-order sum -- no atomics;
+//   * energies go to per-thread partial slots reduced in a fixed order.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace bd {
+
+constexpr double kPI = 3.141592653589790;  // constants.h:6
+constexpr double kLO = 3.0;
+constexpr double kRO = 4.8;
+constexpr double kD1 = 1.8;
+constexpr double kTHETA = 1.2 * kPI / 2;
+constexpr double kETA = 1.137076e-12;
+constexpr double kRH = 1.2;
+constexpr int kMaxGSites = 8;  // globular-head sites per core held in registers
+
+enum Terms { TERM_MECH = 1, TERM_ELEC = 2, TERM_TAIL = 4, TERM_LH = 8 };
+
+// Positions, frames and derived quantities of one configuration.
+struct Geo {
+    double *r, *a, *b, *c;                 // 3n
+    double *alpha, *beta, *gamma, *length; // n
+    double *a_dna, *b_dna, *c_dna;         // 3 n_c
+    double *alpha_p, *beta_p, *gamma_p;    // n_c
+    double *r_t, *beta_t;                  // 3 n_tail, n_tail
+    double *r_lh, *beta_lh;                // 3 n_LH, n_LH
+};
+
+// Per-bead force outputs of one force pass.
+struct Frc {
+    double *F, *T;  // 3n
+    double *tF;     // 3 n_tail
+    double *lF;     // 3 n_LH
+};
+
+// Static device data (everything the step reads but never writes).
+struct Stat {
+    int n, n_c, n_tail, n_t, n_lh_n, n_lh_g, n_lh_c, nlh, n_LH, n_lhc, Nq, n_lnk;
+    const int *type;      // n
+    const int *ic_of;     // n: ordinal of the last core at or before bead i
+    const int *core_bead; // n_c: bead index of core k
+    const int *elig;      // n: bead takes part in the 25 nm core/linker terms
+    const int *lnk;       // n_lnk: bead indices of linker beads
+    const double *phi_o, *rad;
+    const double *cpos, *cq;  // 3Nq, Nq
+    const double *tail_pos;   // 3 n_t
+    const double *h_t, *g_t, *lo_t, *beta_o_t, *t_q, *t_rad;
+    const int *t_grp, *t_fix;
+    const int *t_rcore;       // n_tail: core bead that takes LH forces of an anchored tail
+    const int *rt_off, *rt_list;  // CSR per core ordinal: anchored tails redirected to it
+    const double *LH_g_pos;   // 3 n_lh_g
+    const int *LH_conn;       // template
+    const double *LH_q, *LH_hh, *LH_hc, *LH_hl, *LH_ht, *LH_kstr, *LH_kben, *LH_streq,
+        *LH_betaeq;
+    const int *lhc_bead;      // n_lhc: core bead of LH ordinal k
+    const int *lh_of_core;    // n_c: LH ordinal of core k or -1
+    double h, g, s, debyell, debye, q_l, k_e, k_ex;
+    double sig_dd, sig_dc, sig_cc, sig_tt, sig_tl, sig_tc;
+    double a1, a2, s2dt, del, time_step;
+    double c4pe;      // 4*PI*(1/(4*PI*k_e)) as the reference forms it
+    double rmax_q;    // max |core_pos|
+    double rmax_g;    // max |LH_g_pos|
+};
+
+struct Sites {
+    double *Q;  // 3 * n_c * Nq world positions of core charges
+    double *G;  // 3 * n_lhc * n_lh_g world positions of globular-head sites
+};
+
+__device__ __forceinline__ double3 ld3(const double *p) { return make_double3(p[0], p[1], p[2]); }
+__device__ __forceinline__ void st3(double *p, double3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+__device__ __forceinline__ double3 operator+(double3 u, double3 v) { return make_double3(u.x + v.x, u.y + v.y, u.z + v.z); }
+__device__ __forceinline__ double3 operator-(double3 u, double3 v) { return make_double3(u.x - v.x, u.y - v.y, u.z - v.z); }
+__device__ __forceinline__ double3 operator*(double s, double3 v) { return make_double3(s * v.x, s * v.y, s * v.z); }
+__device__ __forceinline__ void operator+=(double3 &u, double3 v) { u.x += v.x; u.y += v.y; u.z += v.z; }
+__device__ __forceinline__ void operator-=(double3 &u, double3 v) { u.x -= v.x; u.y -= v.y; u.z -= v.z; }
+__device__ __forceinline__ double dot(double3 u, double3 v) { return u.x * v.x + u.y * v.y + u.z * v.z; }
+__device__ __forceinline__ double3 cross(double3 u, double3 v) {
+    return make_double3(u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x);
+}
+__device__ __forceinline__ double3 zero3() { return make_double3(0.0, 0.0, 0.0); }
+
+// r + a*p.x + b*p.y + c*p.z (body point p of a frame at r)
+__device__ __forceinline__ double3 body_to_world(double3 r, double3 a, double3 b, double3 c, double3 p) {
+    return make_double3(r.x + a.x * p.x + b.x * p.y + c.x * p.z, r.y + a.y * p.x + b.y * p.y + c.y * p.z,
+                        r.z + a.z * p.x + b.z * p.y + c.z * p.z);
+}
+
+// torque_due_to_force_relative (func.cpp:396-410): comp x (body components of f)
+__device__ __forceinline__ double3 torque_rel(double3 f, double3 comp, double3 a, double3 b, double3 c) {
+    double fa = dot(a, f), fb = dot(b, f), fc = dot(c, f);
+    return make_double3(fc * comp.y - fb * comp.z, fa * comp.z - fc * comp.x, fb * comp.x - fa * comp.y);
+}
+
+__device__ __forceinline__ double clamp09(double v) {
+    if (v > 0.9) v = 0.9;
+    if (v < -0.9) v = -0.9;
+    return v;
+}
+
+// Pair primitives of func.cpp:278-371 written from the receiver's side:
+// the force on the particle at p from a partner at q is
+// clamp(F * (p - q) / d), which equals the reference's fp1 when the receiver
+// is coord1 and its fp2 when it is coord2 (the clamp is odd-symmetric).
+// Pairs beyond both cut-offs contribute exactly zero and are skipped.
+
+// Force_Ele_Vdw: DH below 7 nm plus excluded volume below 4 nm.
+__device__ __forceinline__ bool dhev_recv(double q1q2, double c4pe, double kappa, double kev,
+                                          double sigma, double3 p, double3 q, double3 &f,
+                                          double &E) {
+    double3 dv = p - q;
+    double d2 = dot(dv, dv);
+    if (!(d2 < 49.01)) return false;
+    double d = sqrt(d2);
+    if (!(d < 7.0)) return false;  // beyond both cut-offs: exactly zero
+    double fs = 0.0;
+    double e = exp(-kappa * d);
+    fs = ((q1q2 * (kappa * d + 1)) / (c4pe * d * d)) * e;
+    E += (q1q2 / (c4pe * d)) * e;
+    if (d < 4.0) {
+        double x = sigma / d, x2 = x * x, x6 = x2 * x2 * x2;
+        double s6 = sigma * sigma, s6b = s6 * s6 * s6;
+        double d7 = d * d2 * d2 * d2;  // d^7
+        fs += -kev * ((6 * s6b) / d7 - (12 * (s6b * s6b)) / (d7 * d2 * d2 * d2));
+        E += kev * (x6 * x6 - x6);
+    }
+    f.x += clamp09(fs * dv.x / d);
+    f.y += clamp09(fs * dv.y / d);
+    f.z += clamp09(fs * dv.z / d);
+    return true;
+}
+
+// Force_Electrostatics: DH below 7 nm only.
+__device__ __forceinline__ bool dh_recv(double q1q2, double c4pe, double kappa, double3 p,
+                                        double3 q, double3 &f, double &E) {
+    double3 dv = p - q;
+    double d2 = dot(dv, dv);
+    if (!(d2 < 49.01)) return false;
+    double d = sqrt(d2);
+    if (!(d < 7.0)) return false;
+    double e = exp(-kappa * d);
+    double fs = ((q1q2 * (kappa * d + 1)) / (c4pe * d * d)) * e;
+    E += (q1q2 / (c4pe * d)) * e;
+    f.x += clamp09(fs * dv.x / d);
+    f.y += clamp09(fs * dv.y / d);
+    f.z += clamp09(fs * dv.z / d);
+    return true;
+}
+
+// Force_Exclude_Volume: below 4 nm only.
+__device__ __forceinline__ bool ev_recv(double kev, double sigma, double3 p, double3 q,
+                                        double3 &f, double &E) {
+    double3 dv = p - q;
+    double d2 = dot(dv, dv);
+    if (!(d2 < 16.01)) return false;
+    double d = sqrt(d2);
+    if (!(d < 4.0)) return false;
+    double x = sigma / d, x2 = x * x, x6 = x2 * x2 * x2;
+    double s6 = sigma * sigma, s6b = s6 * s6 * s6;
+    double d7 = d * d2 * d2 * d2;
+    double fs = -kev * ((6 * s6b) / d7 - (12 * (s6b * s6b)) / (d7 * d2 * d2 * d2));
+    E += kev * (x6 * x6 - x6);
+    f.x += clamp09(fs * dv.x / d);
+    f.y += clamp09(fs * dv.y / d);
+    f.z += clamp09(fs * dv.z / d);
+    return true;
+}
+
+// Force_Stretching (func.cpp:134-151): force on p1 (f1); f2 = -f1.
+__device__ __forceinline__ double3 stretch_f1(double k, double3 p1, double3 p2, double l0,
+                                              double &E) {
+    double3 dv = p1 - p2;
+    double d = sqrt(dot(dv, dv));
+    E += k * (d - l0) * (d - l0) / 2;
+    double f = -k * (d - l0);
+    return make_double3(f * dv.x / d, f * dv.y / d, f * dv.z / d);
+}
+
+// Force_Bending (func.cpp:153-196): the three projections.
+__device__ __forceinline__ void bend3(double kb, double beta, double beta0, double3 p1,
+                                     double3 p2, double3 p3, double3 &f1, double3 &f2,
+                                     double3 &f3, double &E) {
+    E += kb * (beta - beta0) * (beta - beta0) / 2;
+    double f = -kb * (beta - beta0);
+    double3 ji = p2 - p1, jk = p2 - p3, kj = p3 - p2;
+    double d1 = sqrt(dot(ji, ji)), d2 = sqrt(dot(kj, kj));
+    double3 n0 = cross(ji, jk);
+    double3 ri = cross(ji, n0);
+    double nri = sqrt(dot(ri, ri));
+    double3 rk = cross(kj, n0);
+    double nrk = sqrt(dot(rk, rk));
+    double a1 = f / d1, a2 = f / d2;
+    f1 = make_double3(a1 * ri.x / nri, a1 * ri.y / nri, a1 * ri.z / nri);
+    double3 g = make_double3(a2 * rk.x / nrk, a2 * rk.y / nrk, a2 * rk.z / nrk);
+    f2 = make_double3(-a1 * ri.x / nri - g.x, -a1 * ri.y / nri - g.y, -a1 * ri.z / nri - g.z);
+    f3 = g;
+}
+
+// Block-wide sum of a double over blockDim.x threads (fixed order).
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < (NT + 63) / 64; i++) t += sh[i];
+    }
+    __syncthreads();
+    return t;  // valid on thread 0
+}
+
+}  // namespace bd

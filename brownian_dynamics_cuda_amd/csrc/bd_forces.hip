@@ -1,0 +1,797 @@
+// bd_forces.hip -- one force pass of the BD step on gfx950.
+//
+// Restates the CPU force routines of the reference (func.cpp:951-2781) as
+// receiver-side ("gather") kernels: each bead's force and torque is summed
+// by exactly one thread or one workgroup, so the result is a fixed-order
+// FP64 sum with no atomics.  Pairs that contribute to two receivers are
+// evaluated from both sides (cheap short-range terms) except the
+// dominant core-core charge sum, which is evaluated once per core pair
+// into a slot matrix and gathered by bd_cc_reduce.
+//
+// Energies: every pair/bond energy is counted once, by a fixed owner
+// (see the comment at each term), into per-thread slots of Epart.
+#include "bd_dev.h"
+#include "bd_kernels.h"
+
+namespace bd {
+
+// ---------------------------------------------------------------------------
+// world positions of core charges and globular-head sites
+// ---------------------------------------------------------------------------
+__global__ void k_sites(Stat s, Geo g, Sites st) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int nq = s.n_c * s.Nq;
+    if (t < nq) {
+        int k = t / s.Nq, q = t % s.Nq, cb = s.core_bead[k];
+        double3 w = body_to_world(ld3(g.r + 3 * cb), ld3(g.a + 3 * cb), ld3(g.b + 3 * cb),
+                                  ld3(g.c + 3 * cb), ld3(s.cpos + 3 * q));
+        st3(st.Q + 3 * t, w);
+        return;
+    }
+    t -= nq;
+    if (t < s.n_lhc * s.n_lh_g) {
+        int k = t / s.n_lh_g, m = t % s.n_lh_g, cb = s.lhc_bead[k];
+        double3 w = body_to_world(ld3(g.r + 3 * cb), ld3(g.a + 3 * cb), ld3(g.b + 3 * cb),
+                                  ld3(g.c + 3 * cb), ld3(s.LH_g_pos + 3 * m));
+        st3(st.G + 3 * t, w);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// linker-DNA mechanics, mechanical_force_and_torque (func.cpp:951-1289)
+// ---------------------------------------------------------------------------
+struct SegOut {
+    double3 rs, rf;          // exit point of bead i, entry point of bead i+1
+    double3 s1, s2, b1, b2, t1, t2;  // stretch / bend / twist force pairs
+    double E;
+};
+
+__device__ __forceinline__ double3 entry_pt(int t, double3 r, double3 a, double3 b, double3 c) {
+    if (t == 0) return r;
+    const double si = sin(kTHETA), co = cos(kTHETA);
+    double3 bm = make_double3(-si * a.x + co * b.x, -si * a.y + co * b.y, -si * a.z + co * b.z);
+    return make_double3(r.x - kRO * bm.x + kD1 * c.x, r.y - kRO * bm.y + kD1 * c.y,
+                        r.z - kRO * bm.z + kD1 * c.z);
+}
+__device__ __forceinline__ double3 exit_pt(int t, double3 r, double3 b, double3 c) {
+    if (t == 0) return r;
+    return make_double3(r.x - (kRO * b.x + kD1 * c.x), r.y - (kRO * b.y + kD1 * c.y),
+                        r.z - (kRO * b.z + kD1 * c.z));
+}
+
+__device__ void segment(const Stat &s, const Geo &g, int i, SegOut &o) {
+    const double si = sin(kTHETA), co = cos(kTHETA);
+    const int ti = s.type[i], tf = s.type[i + 1];
+    double3 ri = ld3(g.r + 3 * i), ai = ld3(g.a + 3 * i), bi = ld3(g.b + 3 * i), ci = ld3(g.c + 3 * i);
+    double3 rn = ld3(g.r + 3 * i + 3), an = ld3(g.a + 3 * i + 3), bn = ld3(g.b + 3 * i + 3),
+            cn = ld3(g.c + 3 * i + 3);
+    o.rf = entry_pt(tf, rn, an, bn, cn);
+    o.rs = exit_pt(ti, ri, bi, ci);
+    o.E = 0.0;
+    // stretching (func.cpp:978-1018)
+    o.s1 = stretch_f1(s.h, o.rs, o.rf, kLO, o.E);
+    o.s2 = make_double3(-o.s1.x, -o.s1.y, -o.s1.z);
+    const int ic = s.ic_of[i];
+    // bending (func.cpp:1022-1105, Bending_force_projection 198-232)
+    double3 af, ab, ao;
+    double beta_b;
+    if (ti == 0) {
+        af = (tf == 0) ? an : make_double3(co * an.x + si * bn.x, co * an.y + si * bn.y, co * an.z + si * bn.z);
+        ab = (s.type[i - 1] == 0) ? ld3(g.a + 3 * i - 3) : ld3(g.a_dna + 3 * ic);
+        ao = ai;
+        beta_b = g.beta[i - 1];
+    } else {
+        af = an;
+        ab = ai;
+        ao = ld3(g.a_dna + 3 * ic);
+        beta_b = g.beta_p[ic];
+    }
+    {
+        double bb = g.beta[i], len = g.length[i];
+        o.E += s.g * (bb - beta_b) * (bb - beta_b) / 2;
+        double g1 = (bb >= 1e-10) ? bb / (sin(bb) * len) : 1.0 / len;
+        double c1 = cos(bb);
+        double g2 = (beta_b >= 1e-10) ? beta_b / (sin(beta_b) * len) : 1.0 / len;
+        double c2 = cos(beta_b);
+        double3 A = make_double3(g1 * (af.x - c1 * ao.x), g1 * (af.y - c1 * ao.y), g1 * (af.z - c1 * ao.z));
+        double3 B = make_double3(g2 * (ab.x - c2 * ao.x), g2 * (ab.y - c2 * ao.y), g2 * (ab.z - c2 * ao.z));
+        o.b1 = make_double3(-s.g * (A.x + B.x), -s.g * (A.y + B.y), -s.g * (A.z + B.z));
+        o.b2 = make_double3(s.g * (A.x + B.x), s.g * (A.y + B.y), s.g * (A.z + B.z));
+    }
+    // twisting (func.cpp:1110-1179, Twisting_force_projection 234-260)
+    {
+        double al_b, be_b, ga_b, ph_b, ga_n;
+        double3 bo, cc;
+        if (ti == 0) {
+            al_b = g.alpha[i - 1]; be_b = g.beta[i - 1]; ga_b = g.gamma[i - 1];
+            ph_b = s.phi_o[i - 1]; ga_n = g.gamma[i - 1];
+            bo = bi; cc = ci;
+        } else {
+            al_b = g.alpha[i]; be_b = g.beta_p[ic]; ga_b = g.gamma[i];
+            ph_b = s.phi_o[i]; ga_n = g.gamma_p[ic];
+            bo = ld3(g.b_dna + 3 * ic); cc = ld3(g.c_dna + 3 * ic);
+        }
+        double tw = g.alpha[i] + g.gamma[i] - s.phi_o[i];
+        o.E += s.s * (tw * tw) / 2;
+        double g1 = tw * tan(0.5 * g.beta[i]) / g.length[i];
+        double ca = cos(g.alpha[i]), sa = sin(g.alpha[i]);
+        double g2 = (al_b + ga_b - ph_b) * tan(0.5 * be_b) / g.length[i];
+        double cg = cos(ga_n), sg = sin(ga_n);
+        double3 ch = make_double3(g1 * (ca * cc.x - sa * bo.x), g1 * (ca * cc.y - sa * bo.y), g1 * (ca * cc.z - sa * bo.z));
+        double3 zh = make_double3(g2 * (cg * cc.x + sg * bo.x), g2 * (cg * cc.y + sg * bo.y), g2 * (cg * cc.z + sg * bo.z));
+        o.t1 = make_double3(s.s * (ch.x + zh.x), s.s * (ch.y + zh.y), s.s * (ch.z + zh.z));
+        o.t2 = make_double3(-s.s * (ch.x + zh.x), -s.s * (ch.y + zh.y), -s.s * (ch.z + zh.z));
+    }
+}
+
+// torque_due_to_force (func.cpp:374-394) about bead centre pc
+__device__ __forceinline__ double3 torque_abs(double3 f, double3 pf, double3 pc, double3 a, double3 b, double3 c) {
+    double3 d = pf - pc;
+    return torque_rel(f, make_double3(dot(d, a), dot(d, b), dot(d, c)), a, b, c);
+}
+
+// Thread per main bead j: assigns F[j], T[j] (this kernel opens the pass).
+__global__ void k_mech(Stat s, Geo g, Frc f, double *Epart, int terms) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= s.n) return;
+    double3 F = zero3(), T = zero3();
+    double E = 0.0;
+    const int n = s.n;
+    if (terms & TERM_MECH) {
+        double3 rj = ld3(g.r + 3 * j), aj = ld3(g.a + 3 * j), bj = ld3(g.b + 3 * j), cj = ld3(g.c + 3 * j);
+        SegOut L, R;  // segment j-1 (bead j on its right) and segment j
+        bool hasL = j >= 1, hasR = j <= n - 2;
+        if (hasL) segment(s, g, j - 1, L);
+        if (hasR) segment(s, g, j, R);
+        // reference order: all stretches, then bends, then twists
+        if (hasL) { F += L.s2; T += torque_abs(L.s2, L.rf, rj, aj, bj, cj); }
+        if (hasR) { F += R.s1; T += torque_abs(R.s1, R.rs, rj, aj, bj, cj); }
+        if (hasL) { F += L.b2; T += torque_abs(L.b2, L.rf, rj, aj, bj, cj); }
+        if (hasR) { F += R.b1; T += torque_abs(R.b1, R.rs, rj, aj, bj, cj); }
+        if (hasL) { F += L.t2; T += torque_abs(L.t2, L.rf, rj, aj, bj, cj); }
+        if (hasR) { F += R.t1; T += torque_abs(R.t1, R.rs, rj, aj, bj, cj); E = R.E; }
+        const double si = sin(kTHETA), co = cos(kTHETA);
+        const double *al = g.alpha, *ga = g.gamma, *ph = s.phi_o;
+        if (j == n - 1) {
+            // last bead: assigned (func.cpp:1281-1287)
+            T = make_double3(-s.s * (al[n - 2] + ga[n - 2] - ph[n - 2]), 0.0, 0.0);
+        } else if (s.type[j] == 0) {
+            // linker bead: assigned (func.cpp:1206-1209)
+            T = make_double3(s.s * (al[j] + ga[j] + ph[j] - al[j - 1] - ga[j - 1] - ph[j - 1]), 0.0, 0.0);
+        } else {
+            // core: explicit torques (func.cpp:1210-1273)
+            int ic = s.ic_of[j];
+            double3 ad = ld3(g.a_dna + 3 * ic), bd = ld3(g.b_dna + 3 * ic), cd = ld3(g.c_dna + 3 * ic);
+            double ada = dot(ad, aj), adb = dot(ad, bj), adc = dot(ad, cj);
+            double mag = s.s * (al[j] + ga[j] - ph[j]);
+            T.x += mag * ada; T.y += mag * adb; T.z += mag * adc;
+            if (j > 0) {
+                mag = -s.s * (al[j - 1] + ga[j - 1] - ph[j - 1]);
+                T.x += mag * co; T.y += mag * si;
+            }
+            double bp = g.beta_p[ic];
+            T.y = T.y - s.g * bp * adc / sin(bp);
+            T.z = T.z + s.g * bp * adb / sin(bp);
+            if (j > 0) {
+                double3 ab = ld3(g.a + 3 * j - 3);
+                double xa = dot(ab, aj), xb = dot(ab, bj), xc = dot(ab, cj);
+                double bm = g.beta[j - 1];
+                T.x = T.x + s.g * bm * (si * xc) / sin(bm);
+                T.y = T.y - s.g * bm * (co * xc) / sin(bm);
+                T.z = T.z + s.g * bm * (co * xb - si * xa) / sin(bm);
+            }
+            double s1 = sin(g.alpha_p[ic]), c1 = cos(g.alpha_p[ic]);
+            mag = s.s * (al[j] + ga[j] - ph[j]) * tan(0.5 * bp);
+            T.x = T.x - mag * (s1 * dot(cd, aj) + c1 * dot(bd, aj));
+            T.y = T.y - mag * (s1 * dot(cd, bj) + c1 * dot(bd, bj));
+            T.z = T.z - mag * (s1 * dot(cd, cj) + c1 * dot(bd, cj));
+            if (j > 0) {
+                s1 = sin(ga[j - 1]);
+                c1 = cos(ga[j - 1]);
+                mag = s.s * (al[j - 1] + ga[j - 1] - ph[j - 1]) * tan(0.5 * g.beta[j - 1]);
+                T.x = T.x - mag * (s1 * 0.0 - c1 * (-si));
+                T.y = T.y - mag * (s1 * 0.0 - c1 * co);
+                T.z = T.z - mag * (s1 * 1.0 - c1 * 0.0);
+            }
+        }
+    }
+    st3(f.F + 3 * j, F);
+    st3(f.T + 3 * j, T);
+    if (Epart) Epart[j] = E;  // segment j's energy
+}
+
+// ---------------------------------------------------------------------------
+// core / linker nonbonded helpers (func.cpp:1291-1605, no clamps, EV <= 8 nm)
+// ---------------------------------------------------------------------------
+// scalar g1 of the reference for a charge pair at distance d; adds energy
+__device__ __forceinline__ double dh_core(double ke_q1q2, double kappa, double d, double &E) {
+    double e = exp(-kappa * d);
+    E += (ke_q1q2 / d) * e;
+    return ke_q1q2 * e * (kappa * d + 1.0) / (d * d);
+}
+__device__ __forceinline__ double ev_core(double kex, double sg, double d, double &E) {
+    double x = sg / d, x2 = x * x, x6 = x2 * x2 * x2, x7 = x6 * x, x13 = x6 * x7;
+    E += kex * (x6 * x6 - x6);
+    return kex * ((12.0 / sg) * x13 - (6.0 / sg) * x7);
+}
+
+// ---------------------------------------------------------------------------
+// Thread per linker bead: DNA-DNA, DNA-core (linker side), tail-linker,
+// LH-DNA and LHG-DNA (linker side).  F[j] +=.
+// Energy owner: DNA-DNA pairs with partner index > j.
+// ---------------------------------------------------------------------------
+__global__ void k_linker(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
+    int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= s.n_lnk) return;
+    const int j = s.lnk[li];
+    const double3 rj = ld3(g.r + 3 * j);
+    double3 F = zero3();
+    double E = 0.0, Ed = 0.0;
+    if ((terms & TERM_ELEC) && s.elig[j]) {
+        const double kll = s.debyell, qlql = s.q_l * s.q_l;
+        for (int pi = 0; pi < s.n_lnk; pi++) {
+            int i = s.lnk[pi];
+            if (!s.elig[i] || abs(i - j) <= 1) continue;
+            double3 dv = rj - ld3(g.r + 3 * i);
+            double dist = sqrt(dot(dv, dv));
+            if (dist > 25.0) continue;
+            double mi = 1.0 / dist;
+            double Ep = 0.0;
+            double g1 = s.k_e * qlql * exp(-kll * dist) * (kll * dist + 1.0) / (dist * dist);
+            Ep += (s.k_e * qlql / dist) * exp(-kll * dist);
+            if (dist <= 8) g1 += ev_core(s.k_ex, s.sig_dd, dist, Ep);
+            F += (g1 * mi) * dv;
+            if (i > j) E += Ep;
+        }
+        // DNA-core: force on the linker from every charge of eligible cores
+        // within 25 nm (energy owned by the core side)
+        for (int k = 0; k < s.n_c; k++) {
+            int cb = s.core_bead[k];
+            if (!s.elig[cb]) continue;
+            double3 dc = ld3(g.r + 3 * cb) - rj;
+            if (sqrt(dot(dc, dc)) > 25.0) continue;
+            const bool dh = abs(cb - j) > 1;
+            const double *Q = st.Q + 3 * (size_t)k * s.Nq;
+            for (int q = 0; q < s.Nq; q++) {
+                double3 z = rj - ld3(Q + 3 * q);
+                double d = sqrt(dot(z, z));
+                double cq = s.cq[q];
+                double g1 = 0.0;
+                if (dh) g1 += dh_core(s.k_e * s.q_l * cq, s.debye, d, Ed);
+                if (d <= 8.0 && cq > 0) g1 += ev_core(s.k_ex, s.sig_dc, d, Ed);
+                F += (g1 / d) * z;
+            }
+        }
+    }
+    if (terms & TERM_TAIL) {
+        // tail-linker, linker side (func.cpp:1898-1917); energy owned by tails
+        for (int t = 0; t < s.n_tail; t++)
+            dhev_recv(s.t_q[t] * s.q_l, s.c4pe, s.debye, s.k_ex, s.sig_tl, rj, ld3(g.r_t + 3 * t), F, Ed);
+    }
+    if (terms & TERM_LH) {
+        // LH-DNA (func.cpp:2249-2279), energy owned by LH beads
+        for (int i = 0; i < s.n_LH; i++) {
+            int idx = i % s.nlh;
+            idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
+            dhev_recv(s.LH_q[idx] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[idx], rj, ld3(g.r_lh + 3 * i), F, Ed);
+        }
+        // LHG-DNA (func.cpp:2481-2522), energy owned by the core side
+        for (int k = 0; k < s.n_lhc; k++)
+            for (int m = 0; m < s.n_lh_g; m++)
+                dhev_recv(s.LH_q[m] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[m], rj,
+                          ld3(st.G + 3 * (k * s.n_lh_g + m)), F, Ed);
+    }
+    double *Fj = f.F + 3 * j;
+    Fj[0] += F.x; Fj[1] += F.y; Fj[2] += F.z;
+    if (Epart) Epart[li] = E;
+}
+
+// ---------------------------------------------------------------------------
+// Thread per tail bead (func.cpp:1793-2026 and the tail side of the LH
+// terms).  Assigns tF[j].  Energy owner: bonds / angles starting at j,
+// tail-tail pairs with partner > j, tail-linker pairs, the anchor spring.
+// ---------------------------------------------------------------------------
+__global__ void k_tail(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= s.n_tail) return;
+    const int nt = s.n_tail;
+    const int *tg = s.t_grp;
+    const double3 rj = ld3(g.r_t + 3 * j);
+    double3 F = zero3();
+    double E = 0.0, Ed = 0.0;
+    if (terms & TERM_TAIL) {
+        // stretching of bonds (j-1,j) and (j,j+1)
+        if (j >= 1 && tg[j - 1] == tg[j]) {
+            double3 f1 = stretch_f1(s.h_t[j - 1], ld3(g.r_t + 3 * j - 3), rj, s.lo_t[j - 1], Ed);
+            F -= f1;
+        }
+        if (j + 1 < nt && tg[j] == tg[j + 1]) F += stretch_f1(s.h_t[j], rj, ld3(g.r_t + 3 * j + 3), s.lo_t[j], E);
+        // bending of triples starting at j-2, j-1, j (forces subtracted)
+        for (int c0 = j - 2; c0 <= j; c0++) {
+            if (c0 < 0 || c0 + 2 >= nt) continue;
+            if (!(tg[c0] == tg[c0 + 1] && tg[c0] == tg[c0 + 2])) continue;
+            double3 f1, f2, f3;
+            bend3(s.g_t[c0], g.beta_t[c0], s.beta_o_t[c0], ld3(g.r_t + 3 * c0), ld3(g.r_t + 3 * c0 + 3),
+                  ld3(g.r_t + 3 * c0 + 6), f1, f2, f3, (c0 == j) ? E : Ed);
+            F -= (c0 == j) ? f1 : (c0 == j - 1 ? f2 : f3);
+        }
+        // tail-tail DH (func.cpp:1863-1891), skipping same-group neighbours
+        const double qj = s.t_q[j];
+        for (int k = 0; k < nt; k++) {
+            if (k == j) continue;
+            if (!(tg[k] != tg[j] || abs(k - j) > 2)) continue;
+            dh_recv(qj * s.t_q[k], s.c4pe, s.debye, rj, ld3(g.r_t + 3 * k), F, (k > j) ? E : Ed);
+        }
+        // tail-linker (func.cpp:1898-1917)
+        for (int pi = 0; pi < s.n_lnk; pi++)
+            dhev_recv(qj * s.q_l, s.c4pe, s.debye, s.k_ex, s.sig_tl, rj, ld3(g.r + 3 * s.lnk[pi]), F, E);
+        // anchor spring to the owning core (func.cpp:1971-2022)
+        if (s.t_fix[j] == 1) {
+            int cb = s.core_bead[j / s.n_t];
+            double3 ro = body_to_world(ld3(g.r + 3 * cb), ld3(g.a + 3 * cb), ld3(g.b + 3 * cb),
+                                       ld3(g.c + 3 * cb), ld3(s.tail_pos + 3 * (j % s.n_t)));
+            double3 S = rj - ro;
+            E += s.h * (S.x * S.x + S.y * S.y + S.z * S.z) / 2;
+            F -= make_double3(s.h * S.x, s.h * S.y, s.h * S.z);
+        }
+    }
+    if ((terms & TERM_LH) && s.t_fix[j] == 0) {
+        // LH-tail (func.cpp:2343-2398) and LHG-tail (2403-2476), tail side;
+        // anchored tails pass these forces to a core (k_core)
+        for (int i = 0; i < s.n_LH; i++) {
+            int idx = i % s.nlh;
+            idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
+            dhev_recv(s.LH_q[idx] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rj, ld3(g.r_lh + 3 * i), F, Ed);
+        }
+        for (int k = 0; k < s.n_lhc; k++)
+            for (int m = 0; m < s.n_lh_g; m++)
+                dhev_recv(s.LH_q[m] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], rj,
+                          ld3(st.G + 3 * (k * s.n_lh_g + m)), F, Ed);
+    }
+    st3(f.tF + 3 * j, F);
+    if (Epart) Epart[j] = E;
+}
+
+// ---------------------------------------------------------------------------
+// Thread per free LH bead (func.cpp:2093-2781, LH side).  Assigns lF[j].
+// Energy owner: bonds / angles at j, LH-LH pairs with partner > j, LH-DNA,
+// LH-tail.
+// ---------------------------------------------------------------------------
+__global__ void k_lh(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= s.n_LH) return;
+    const int nlh = s.nlh;
+    auto tpl = [&](int i) { int x = i % nlh; return (x < s.n_lh_n) ? x : x + s.n_lh_g; };
+    const int idx = tpl(j);
+    const double3 rj = ld3(g.r_lh + 3 * j);
+    double3 F = zero3();
+    double E = 0.0, Ed = 0.0;
+    if (terms & TERM_LH) {
+        // stretching (func.cpp:2111-2143)
+        if (j >= 1 && s.LH_conn[tpl(j - 1)] == 1) {
+            int ip = tpl(j - 1);
+            F -= stretch_f1(s.LH_kstr[ip], ld3(g.r_lh + 3 * j - 3), rj, s.LH_streq[ip], Ed);
+        }
+        if (j + 1 < s.n_LH && s.LH_conn[idx] == 1) F += stretch_f1(s.LH_kstr[idx], rj, ld3(g.r_lh + 3 * j + 3), s.LH_streq[idx], E);
+        // bending centred at j-1, j, j+1 (func.cpp:2149-2181)
+        for (int c0 = j - 1; c0 <= j + 1; c0++) {
+            if (c0 < 1 || c0 + 1 >= s.n_LH) continue;
+            if (!((c0 % s.n_lh_c != 0) && (c0 % s.n_lh_c != s.n_lh_c - 1))) continue;
+            int ic = tpl(c0);
+            double3 f1, f2, f3;
+            bend3(s.LH_kben[ic], g.beta_lh[c0], s.LH_betaeq[ic], ld3(g.r_lh + 3 * c0 - 3),
+                  ld3(g.r_lh + 3 * c0), ld3(g.r_lh + 3 * c0 + 3), f1, f2, f3, (c0 == j) ? E : Ed);
+            F -= (c0 == j + 1) ? f1 : (c0 == j ? f2 : f3);
+        }
+        // LH-LH (func.cpp:2190-2244)
+        const int mine = j / nlh;
+        for (int k = 0; k < s.n_LH; k++) {
+            if (k == j) continue;
+            int i2 = tpl(k);
+            double sg = (s.LH_hh[idx] + s.LH_hh[i2]) / 2;
+            double &Ek = (k > j) ? E : Ed;
+            if (k / nlh == mine) {
+                if (abs(k - j) > 1) ev_recv(s.k_ex, sg, rj, ld3(g.r_lh + 3 * k), F, Ek);
+            } else {
+                dhev_recv(s.LH_q[idx] * s.LH_q[i2], s.c4pe, s.debye, s.k_ex, sg, rj, ld3(g.r_lh + 3 * k), F, Ek);
+            }
+        }
+        // LH-DNA (func.cpp:2249-2279)
+        for (int pi = 0; pi < s.n_lnk; pi++)
+            dhev_recv(s.LH_q[idx] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[idx], rj, ld3(g.r + 3 * s.lnk[pi]), F, E);
+        // LH-tail (func.cpp:2343-2398)
+        for (int t = 0; t < s.n_tail; t++)
+            dhev_recv(s.LH_q[idx] * s.t_q[t], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rj, ld3(g.r_t + 3 * t), F, E);
+        // LHG of other cores - this bead (func.cpp:2590-2644), energy on the core side
+        for (int k = 0; k < s.n_lhc; k++) {
+            if (k == mine) continue;
+            for (int m = 0; m < s.n_lh_g; m++)
+                dhev_recv(s.LH_q[m] * s.LH_q[idx], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[idx]) / 2, rj,
+                          ld3(st.G + 3 * (k * s.n_lh_g + m)), F, Ed);
+        }
+        // globular head - first C-term bead spring: force x10 (func.cpp:2727-2777)
+        if (j % nlh == 0) {
+            for (int m = 0; m < s.n_lh_g; m++) {
+                int ti = m + s.n_lh_n;
+                if (s.LH_conn[ti] != 1) continue;
+                double3 G = ld3(st.G + 3 * (mine * s.n_lh_g + m));
+                double3 dv = rj - G;
+                double d = sqrt(dot(dv, dv));
+                double ks = s.LH_kstr[ti], l0 = s.LH_streq[ti];
+                double3 S = make_double3((d - l0) * dv.x / d, (d - l0) * dv.y / d, (d - l0) * dv.z / d);
+                F -= make_double3(ks * S.x * 10, ks * S.y * 10, ks * S.z * 10);
+            }
+        }
+    }
+    st3(f.lF + 3 * j, F);
+    if (Epart) Epart[j] = E;
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup per core (ordinal kc): everything acting on the core body except
+// the core-core charge sum.  F[cb] +=, T[cb] +=.
+// Energy owner: core-linker charge pairs, LHG-* pairs of this core's head
+// (LHG-LHG only towards higher LH ordinals), the LHG-C spring.
+// ---------------------------------------------------------------------------
+constexpr int kCoreThreads = 320;
+constexpr int kMaxG = 8;
+
+__global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
+    const int kc = blockIdx.x;
+    const int cb = s.core_bead[kc];
+    const int tid = threadIdx.x;
+    const double3 rc = ld3(g.r + 3 * cb), ac = ld3(g.a + 3 * cb), bc = ld3(g.b + 3 * cb), cc = ld3(g.c + 3 * cb);
+    double3 F = zero3(), T = zero3();
+    double E = 0.0, Ed = 0.0;
+    const double *Qc = st.Q + 3 * (size_t)kc * s.Nq;
+    const int lho = s.lh_of_core[kc];
+
+    // --- charge-owned terms: thread per charge q of this core ---
+    for (int q = tid; q < s.Nq; q += blockDim.x) {
+        const double3 Qq = ld3(Qc + 3 * q);
+        const double cq = s.cq[q];
+        double3 Fq = zero3();
+        if ((terms & TERM_ELEC) && s.elig[cb]) {
+            // core-linker (func.cpp:1356-1487): force on the charge
+            for (int pi = 0; pi < s.n_lnk; pi++) {
+                int L = s.lnk[pi];
+                if (!s.elig[L]) continue;
+                double3 rl = ld3(g.r + 3 * L);
+                double3 dc = rc - rl;
+                if (sqrt(dot(dc, dc)) > 25.0) continue;
+                double3 z = Qq - rl;
+                double d = sqrt(dot(z, z));
+                double g1 = 0.0;
+                if (abs(cb - L) > 1) g1 += dh_core(s.k_e * s.q_l * cq, s.debye, d, E);
+                if (d <= 8.0 && cq > 0) g1 += ev_core(s.k_ex, s.sig_dc, d, E);
+                Fq += (g1 / d) * z;
+            }
+        }
+        if (terms & TERM_LH) {
+            // LHG of other cores acting on this charge (func.cpp:2526-2586, charge side)
+            for (int k = 0; k < s.n_lhc; k++) {
+                int ob = s.lhc_bead[k];
+                if (ob == cb) continue;
+                double3 dc = ld3(g.r + 3 * ob) - rc;
+                if (sqrt(dot(dc, dc)) - s.rmax_g - s.rmax_q > 7.0 + 1e-6) continue;
+                for (int m = 0; m < s.n_lh_g; m++)
+                    dhev_recv(s.LH_q[m] * cq, s.c4pe, s.debye, s.k_ex, s.LH_hc[m], Qq,
+                              ld3(st.G + 3 * (k * s.n_lh_g + m)), Fq, Ed);
+            }
+        }
+        F += Fq;
+        T += torque_rel(Fq, ld3(s.cpos + 3 * q), ac, bc, cc);
+    }
+
+    // --- own globular-head sites (LH-bound cores) ---
+    if ((terms & TERM_LH) && lho >= 0) {
+        const int ng = s.n_lh_g;
+        double3 Gm[kMaxGSites], Fm[kMaxGSites];
+        for (int m = 0; m < ng; m++) { Gm[m] = ld3(st.G + 3 * (lho * ng + m)); Fm[m] = zero3(); }
+        // LHG-tail (2403-2476)
+        for (int t = tid; t < s.n_tail; t += blockDim.x) {
+            double3 rt = ld3(g.r_t + 3 * t);
+            for (int m = 0; m < ng; m++)
+                dhev_recv(s.LH_q[m] * s.t_q[t], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], Gm[m], rt, Fm[m], E);
+        }
+        // LHG-DNA (2481-2522)
+        for (int pi = tid; pi < s.n_lnk; pi += blockDim.x) {
+            double3 rl = ld3(g.r + 3 * s.lnk[pi]);
+            for (int m = 0; m < ng; m++)
+                dhev_recv(s.LH_q[m] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[m], Gm[m], rl, Fm[m], E);
+        }
+        // LHG-charges of every other core (2526-2586), culled per core
+        for (int k = 0; k < s.n_c; k++) {
+            int ob = s.core_bead[k];
+            if (ob == cb) continue;
+            double3 dc = ld3(g.r + 3 * ob) - rc;
+            if (sqrt(dot(dc, dc)) - s.rmax_g - s.rmax_q > 7.0 + 1e-6) continue;
+            const double *Qk = st.Q + 3 * (size_t)k * s.Nq;
+            for (int q = tid; q < s.Nq; q += blockDim.x) {
+                double3 Qq = ld3(Qk + 3 * q);
+                for (int m = 0; m < ng; m++)
+                    dhev_recv(s.LH_q[m] * s.cq[q], s.c4pe, s.debye, s.k_ex, s.LH_hc[m], Gm[m], Qq, Fm[m], E);
+            }
+        }
+        // LHG-free LH beads of other cores (2590-2644)
+        for (int jl = tid; jl < s.n_LH; jl += blockDim.x) {
+            if (jl / s.nlh == lho) continue;
+            int i2 = jl % s.nlh;
+            i2 = (i2 < s.n_lh_n) ? i2 : i2 + s.n_lh_g;
+            double3 rl = ld3(g.r_lh + 3 * jl);
+            for (int m = 0; m < ng; m++)
+                dhev_recv(s.LH_q[m] * s.LH_q[i2], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[i2]) / 2, Gm[m], rl, Fm[m], E);
+        }
+        // LHG-LHG with the other LH-bound cores (2649-2719)
+        for (int x = tid; x < s.n_lhc * ng; x += blockDim.x) {
+            int k = x / ng, m1 = x % ng;
+            if (k == lho) continue;
+            double3 G2 = ld3(st.G + 3 * x);
+            for (int m = 0; m < ng; m++)
+                dhev_recv(s.LH_q[m] * s.LH_q[m1], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[m1]) / 2, Gm[m], G2, Fm[m],
+                          (k > lho) ? E : Ed);
+        }
+        // head - first C-term bead spring (2727-2777): force x10, torque x1
+        if (tid == 0 && s.n_LH > 0) {
+            double3 p = ld3(g.r_lh + 3 * (size_t)lho * s.nlh);
+            for (int m = 0; m < ng; m++) {
+                int ti = m + s.n_lh_n;
+                if (s.LH_conn[ti] != 1) continue;
+                double3 dv = p - Gm[m];
+                double d = sqrt(dot(dv, dv));
+                double ks = s.LH_kstr[ti], l0 = s.LH_streq[ti];
+                E += ks * (d - l0) * (d - l0) / 2;
+                double3 S = make_double3((d - l0) * dv.x / d, (d - l0) * dv.y / d, (d - l0) * dv.z / d);
+                F += make_double3(ks * S.x * 10, ks * S.y * 10, ks * S.z * 10);
+                T += torque_rel(make_double3(ks * S.x, ks * S.y, ks * S.z), ld3(s.LH_g_pos + 3 * m), ac, bc, cc);
+            }
+        }
+        for (int m = 0; m < ng; m++) {
+            F += Fm[m];
+            T += torque_rel(Fm[m], ld3(s.LH_g_pos + 3 * m), ac, bc, cc);
+        }
+    }
+
+    // --- anchored tails whose LH forces are redirected to this core ---
+    __shared__ double red[16 * 8];
+    if (terms & TERM_LH) {
+        for (int e = s.rt_off[kc]; e < s.rt_off[kc + 1]; e++) {
+            const int tj = s.rt_list[e];
+            const double3 rt = ld3(g.r_t + 3 * tj);
+            double3 Ft = zero3();
+            for (int i = tid; i < s.n_LH; i += blockDim.x) {
+                int idx = i % s.nlh;
+                idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
+                dhev_recv(s.LH_q[idx] * s.t_q[tj], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rt, ld3(g.r_lh + 3 * i), Ft, Ed);
+            }
+            for (int x = tid; x < s.n_lhc * s.n_lh_g; x += blockDim.x) {
+                int m = x % s.n_lh_g;
+                dhev_recv(s.LH_q[m] * s.t_q[tj], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], rt, ld3(st.G + 3 * x), Ft, Ed);
+            }
+            double fx = block_sum<kCoreThreads>(Ft.x, red);
+            double fy = block_sum<kCoreThreads>(Ft.y, red);
+            double fz = block_sum<kCoreThreads>(Ft.z, red);
+            if (tid == 0) {
+                double3 ff = make_double3(fx, fy, fz);
+                F += ff;
+                T += torque_abs(ff, rt, rc, ac, bc, cc);
+            }
+        }
+    }
+    // --- anchor springs of this core's own tails (1971-2022); energy on the tail side ---
+    if ((terms & TERM_TAIL) && tid == 0 && s.n_tail > 0) {
+        for (int jj = 0; jj < s.n_t; jj++) {
+            int tj = kc * s.n_t + jj;
+            if (s.t_fix[tj] != 1) continue;
+            double3 ro = body_to_world(rc, ac, bc, cc, ld3(s.tail_pos + 3 * jj));
+            double3 S = ld3(g.r_t + 3 * tj) - ro;
+            double3 hs = make_double3(s.h * S.x, s.h * S.y, s.h * S.z);
+            F += hs;
+            T += torque_rel(hs, ld3(s.tail_pos + 3 * jj), ac, bc, cc);
+        }
+    }
+    // block reduction in fixed order
+    double v[7] = {F.x, F.y, F.z, T.x, T.y, T.z, E};
+    double out[7];
+    for (int q = 0; q < 7; q++) out[q] = block_sum<kCoreThreads>(v[q], red);
+    if (tid == 0) {
+        double *Fc = f.F + 3 * cb, *Tc = f.T + 3 * cb;
+        Fc[0] += out[0]; Fc[1] += out[1]; Fc[2] += out[2];
+        Tc[0] += out[3]; Tc[1] += out[4]; Tc[2] += out[5];
+        if (Epart) Epart[kc] = out[6];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// core-core charge sums (func.cpp:1489-1566)
+// ---------------------------------------------------------------------------
+// Flags core pairs (k1 < k2) inside 25 nm with both cores eligible and
+// appends them to a list; the slot matrix is indexed by (k1, k2), so the
+// list order does not affect the result.
+__global__ void k_cc_list(Stat s, Geo g, unsigned char *valid, int *list, int *count) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long nc = s.n_c;
+    if (t >= nc * nc) return;
+    int k1 = (int)(t / nc), k2 = (int)(t % nc);
+    unsigned char v = 0;
+    if (k1 < k2) {
+        int b1 = s.core_bead[k1], b2 = s.core_bead[k2];
+        if (s.elig[b1] && s.elig[b2]) {
+            double3 dv = ld3(g.r + 3 * b2) - ld3(g.r + 3 * b1);
+            if (!(sqrt(dot(dv, dv)) > 25.0)) v = 1;
+        }
+    }
+    valid[t] = v;
+    if (v) {
+        int pos = atomicAdd(count, 1);
+        list[2 * pos] = k1;
+        list[2 * pos + 1] = k2;
+    }
+}
+
+constexpr int kCCThreads = 320;
+constexpr int kCCChunkMax = 160;
+
+// Work unit = (pair, chunk of the second core's charges).  Thread = charge
+// of the first core.  Slot = {F on c1 (3), T on c1 body (3), T on c2 world
+// (3), E}.  The torque on c2 is summed in world axes about its centre and
+// turned into body components by k_cc_reduce (equal to the reference's
+// body-frame sum for an orthonormal frame).
+__global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, Sites st, const int *list,
+                                                       const int *count, int nchunk, double *slots) {
+    __shared__ double sq[kCCChunkMax * 7];
+    __shared__ double red[16];
+    const int nunits = (*count) * nchunk;
+    const int chunk = (s.Nq + nchunk - 1) / nchunk;
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const int pr = u / nchunk, ch = u % nchunk;
+        const int k1 = list[2 * pr], k2 = list[2 * pr + 1];
+        const int b1 = s.core_bead[k1], b2 = s.core_bead[k2];
+        const int l0 = ch * chunk, l1 = min(s.Nq, l0 + chunk);
+        const double3 r2 = ld3(g.r + 3 * b2);
+        __syncthreads();
+        for (int l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
+            double3 Q = ld3(st.Q + 3 * ((size_t)k2 * s.Nq + l));
+            double *d = sq + 7 * (l - l0);
+            d[0] = Q.x; d[1] = Q.y; d[2] = Q.z; d[3] = s.cq[l];
+            d[4] = Q.x - r2.x; d[5] = Q.y - r2.y; d[6] = Q.z - r2.z;
+        }
+        __syncthreads();
+        double3 F = zero3(), T1 = zero3(), T2 = zero3();
+        double E = 0.0;
+        const double3 a1 = ld3(g.a + 3 * b1), bb1 = ld3(g.b + 3 * b1), c1 = ld3(g.c + 3 * b1);
+        for (int k = threadIdx.x; k < s.Nq; k += blockDim.x) {
+            const double3 Qk = ld3(st.Q + 3 * ((size_t)k1 * s.Nq + k));
+            const double keq = s.k_e * s.cq[k];
+            double3 Fk = zero3();
+            for (int l = 0; l < l1 - l0; l++) {
+                const double *d = sq + 7 * l;
+                double3 z = make_double3(Qk.x - d[0], Qk.y - d[1], Qk.z - d[2]);
+                double dist = sqrt(dot(z, z));
+                double g1 = dh_core(keq * d[3], s.debye, dist, E);
+                if (dist <= 8.0) g1 += ev_core(s.k_ex, s.sig_cc, dist, E);
+                double3 fk = (g1 / dist) * z;
+                Fk += fk;
+                // torque on c2 about its centre from -fk at lever (Q_l - r2)
+                T2.x -= d[5] * fk.z - d[6] * fk.y;
+                T2.y -= d[6] * fk.x - d[4] * fk.z;
+                T2.z -= d[4] * fk.y - d[5] * fk.x;
+            }
+            F += Fk;
+            T1 += torque_rel(Fk, ld3(s.cpos + 3 * k), a1, bb1, c1);
+        }
+        double v[10] = {F.x, F.y, F.z, T1.x, T1.y, T1.z, T2.x, T2.y, T2.z, E};
+        double *slot = slots + 10 * ((size_t)(k1 * s.n_c + k2) * nchunk + ch);
+        for (int q = 0; q < 10; q++) {
+            double t = block_sum<kCCThreads>(v[q], red);
+            if (threadIdx.x == 0) slot[q] = t;
+        }
+    }
+}
+
+// Thread per core: gathers its row (as c1) and column (as c2) of the slot
+// matrix in index order.  F +=, T +=.  Energy owner: the lower index.
+__global__ void k_cc_reduce(Stat s, Geo g, const unsigned char *valid, int nchunk, const double *slots, Frc f,
+                            double *Epart) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.n_c) return;
+    const int cb = s.core_bead[k];
+    double3 F = zero3(), T = zero3(), Tw = zero3();
+    double E = 0.0;
+    for (int o = 0; o < s.n_c; o++) {
+        if (o == k) continue;
+        const bool first = k < o;
+        const size_t pidx = first ? (size_t)k * s.n_c + o : (size_t)o * s.n_c + k;
+        if (!valid[pidx]) continue;
+        for (int ch = 0; ch < nchunk; ch++) {
+            const double *sl = slots + 10 * (pidx * nchunk + ch);
+            if (first) {
+                F += make_double3(sl[0], sl[1], sl[2]);
+                T += make_double3(sl[3], sl[4], sl[5]);
+                E += sl[9];
+            } else {
+                F -= make_double3(sl[0], sl[1], sl[2]);
+                Tw += make_double3(sl[6], sl[7], sl[8]);
+            }
+        }
+    }
+    const double3 a = ld3(g.a + 3 * cb), b = ld3(g.b + 3 * cb), c = ld3(g.c + 3 * cb);
+    T += make_double3(dot(a, Tw), dot(b, Tw), dot(c, Tw));
+    double *Fc = f.F + 3 * cb, *Tc = f.T + 3 * cb;
+    Fc[0] += F.x; Fc[1] += F.y; Fc[2] += F.z;
+    Tc[0] += T.x; Tc[1] += T.y; Tc[2] += T.z;
+    if (Epart) Epart[k] = E;
+}
+
+// Energy partials -> one double, fixed order (one workgroup).
+__global__ void k_ereduce(const double *Epart, int ne, double *out) {
+    __shared__ double red[16];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < ne; i += blockDim.x) v += Epart[i];
+    double t = block_sum<1024>(v, red);
+    if (threadIdx.x == 0) out[0] = t;
+}
+
+__global__ void k_fill(double *p, long long n, double v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_i(int *p, long long n, int v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
+
+void launch_sites(const Stat &s, const Geo &g, const Sites &st, hipStream_t q) {
+    long long tot = (long long)s.n_c * s.Nq + (long long)s.n_lhc * s.n_lh_g;
+    if (tot > 0) k_sites<<<nblk(tot, 256), 256, 0, q>>>(s, g, st);
+}
+
+void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, const ForceScratch &w,
+                   double *Epart, int terms, hipStream_t q, MarkFn mark, void *arg) {
+    const int off_mech = 0, off_lnk = s.n, off_tail = off_lnk + s.n_lnk, off_lh = off_tail + s.n_tail,
+              off_core = off_lh + s.n_LH, off_cc = off_core + s.n_c;
+    auto E = [&](int off) { return Epart ? Epart + off : nullptr; };
+    auto M = [&](int id, int ph) { if (mark) mark(arg, id, ph); };
+    M(K_SITES, 0); launch_sites(s, g, st, q); M(K_SITES, 1);
+    M(K_MECH, 0); k_mech<<<nblk(s.n, 64), 64, 0, q>>>(s, g, f, E(off_mech), terms); M(K_MECH, 1);
+    if (s.n_lnk) { M(K_LINKER, 0); k_linker<<<nblk(s.n_lnk, 64), 64, 0, q>>>(s, g, st, f, E(off_lnk), terms); M(K_LINKER, 1); }
+    if (s.n_tail) { M(K_TAIL, 0); k_tail<<<nblk(s.n_tail, 64), 64, 0, q>>>(s, g, st, f, E(off_tail), terms); M(K_TAIL, 1); }
+    if (s.n_LH) { M(K_LH, 0); k_lh<<<nblk(s.n_LH, 64), 64, 0, q>>>(s, g, st, f, E(off_lh), terms); M(K_LH, 1); }
+    M(K_CORE, 0); k_core<<<s.n_c, kCoreThreads, 0, q>>>(s, g, st, f, E(off_core), terms); M(K_CORE, 1);
+    if (terms & TERM_ELEC) {
+        long long nn = (long long)s.n_c * s.n_c;
+        M(K_CCLIST, 0);
+        k_fill_i<<<1, 64, 0, q>>>(w.cc_count, 1, 0);
+        k_cc_list<<<nblk(nn, 256), 256, 0, q>>>(s, g, w.cc_valid, w.cc_list, w.cc_count);
+        M(K_CCLIST, 1);
+        M(K_CCWORK, 0);
+        k_cc_work<<<w.cc_grid, kCCThreads, 0, q>>>(s, g, st, w.cc_list, w.cc_count, w.cc_nchunk, w.cc_slots);
+        M(K_CCWORK, 1);
+        M(K_CCRED, 0);
+        k_cc_reduce<<<nblk(s.n_c, 64), 64, 0, q>>>(s, g, w.cc_valid, w.cc_nchunk, w.cc_slots, f, E(off_cc));
+        M(K_CCRED, 1);
+    } else if (Epart) {
+        k_fill<<<nblk(s.n_c, 64), 64, 0, q>>>(Epart + off_cc, s.n_c, 0.0);
+    }
+}
+
+int energy_slots(const Stat &s) { return s.n + s.n_lnk + s.n_tail + s.n_LH + 2 * s.n_c; }
+
+void launch_energy_reduce(const double *Epart, int ne, double *out, hipStream_t q) {
+    k_ereduce<<<1, 1024, 0, q>>>(Epart, ne, out);
+}
+
+void launch_fill(double *p, long long n, double v, hipStream_t q) {
+    if (n > 0) k_fill<<<nblk(n, 256), 256, 0, q>>>(p, n, v);
+}
+
+void launch_fill_int(int *p, long long n, int v, hipStream_t q) {
+    if (n > 0) k_fill_i<<<nblk(n, 256), 256, 0, q>>>(p, n, v);
+}
+
+}  // namespace bd

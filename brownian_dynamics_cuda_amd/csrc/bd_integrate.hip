@@ -1,0 +1,347 @@
+// bd_integrate.hip -- frames, Euler angles, rotation, translation, Heun
+// average and commit of the BD step (gfx950).
+//
+// Frame and angle kernels keep the reference's operation order with fused
+// multiply-add contraction switched off: the Euler angles come out of
+// acos() of dot products that sit at +-1 for straight linkers, where any
+// rounding change is amplified to ~1e-8.
+#include "bd_dev.h"
+#include "bd_kernels.h"
+
+namespace bd {
+
+static inline int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
+
+__device__ __forceinline__ double clamp1(double v) {
+    if (v > 1.0) v = 1.0;
+    if (v < -1.0) v = -1.0;
+    return v;
+}
+
+// update_Euler_Angle pass 1 (func.cpp:448-505): linker frames and lengths.
+// Only linker frames are written and only core frames are read across
+// beads, so one thread per bead has no race (the reference GPU kernel
+// fuses both passes and races, func_cuda.cu:722-957).
+__global__ void k_euler1(Stat s, Geo g) {
+#pragma clang fp contract(off)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n - 1) return;
+    const double si = sin(kTHETA), co = cos(kTHETA);
+    double *ri = g.r + 3 * i, *rf = ri + 3, *ai = g.a + 3 * i, *bi = g.b + 3 * i, *ci = g.c + 3 * i;
+    const double *af = ai + 3, *bf = bi + 3, *cf = ci + 3;
+    double fw[3];
+    if (s.type[i] == 0) {
+        double ao[3] = {ai[0], ai[1], ai[2]};
+        if (s.type[i + 1] == 0) {
+            for (int k = 0; k < 3; k++) fw[k] = rf[k] - ri[k];
+        } else {
+            for (int k = 0; k < 3; k++) {
+                double bm = -si * af[k] + co * bf[k];
+                fw[k] = rf[k] - kRO * bm + kD1 * cf[k] - ri[k];
+            }
+        }
+        double len = sqrt(fw[0] * fw[0] + fw[1] * fw[1] + fw[2] * fw[2]);
+        g.length[i] = len;
+        double mi = 1.0 / len;
+        double an[3] = {mi * fw[0], mi * fw[1], mi * fw[2]};
+        double da[3] = {an[0] - ao[0], an[1] - ao[1], an[2] - ao[2]};
+        double b0 = bi[0], b1 = bi[1], b2 = bi[2];
+        double bda = b0 * da[0] + b1 * da[1] + b2 * da[2];
+        b0 = b0 - bda * ao[0]; b1 = b1 - bda * ao[1]; b2 = b2 - bda * ao[2];
+        bda = b0 * an[0] + b1 * an[1] + b2 * an[2];
+        b0 = b0 - bda * an[0]; b1 = b1 - bda * an[1]; b2 = b2 - bda * an[2];
+        mi = 1.0 / sqrt(b0 * b0 + b1 * b1 + b2 * b2);
+        b0 = mi * b0; b1 = mi * b1; b2 = mi * b2;
+        ai[0] = an[0]; ai[1] = an[1]; ai[2] = an[2];
+        bi[0] = b0; bi[1] = b1; bi[2] = b2;
+        ci[0] = an[1] * b2 - an[2] * b1;
+        ci[1] = an[2] * b0 - an[0] * b2;
+        ci[2] = an[0] * b1 - an[1] * b0;
+    } else {
+        for (int k = 0; k < 3; k++) fw[k] = rf[k] - (ri[k] - kRO * bi[k] - kD1 * ci[k]);
+        g.length[i] = sqrt(fw[0] * fw[0] + fw[1] * fw[1] + fw[2] * fw[2]);
+    }
+}
+
+__device__ __forceinline__ double d3(const double *u, const double *v) {
+#pragma clang fp contract(off)
+    return u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+}
+__device__ __forceinline__ double d6(const double *u, const double *v, const double *w, const double *x) {
+#pragma clang fp contract(off)
+    return u[0] * v[0] + u[1] * v[1] + u[2] * v[2] + w[0] * x[0] + w[1] * x[1] + w[2] * x[2];
+}
+
+// update_Euler_Angle pass 2 (func.cpp:507-681): Euler angles and the exit
+// frame of every core.
+__global__ void k_euler2(Stat s, Geo g) {
+#pragma clang fp contract(off)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n - 1) return;
+    const double si = sin(kTHETA), co = cos(kTHETA);
+    const double *ri = g.r + 3 * i, *rf = ri + 3, *ai = g.a + 3 * i, *bi = g.b + 3 * i, *ci = g.c + 3 * i;
+    const double *af = ai + 3, *bf = bi + 3, *cf = ci + 3;
+    double ada, f1, f2, Ac, apg, sb;
+    if (s.type[i] == 0) {
+        double am[3], bm[3];
+        if (s.type[i + 1] == 0) {
+            for (int k = 0; k < 3; k++) { am[k] = af[k]; bm[k] = bf[k]; }
+        } else {
+            for (int k = 0; k < 3; k++) {
+                am[k] = co * af[k] + si * bf[k];
+                bm[k] = -si * af[k] + co * bf[k];
+            }
+        }
+        ada = clamp1(d3(ai, am));
+        double be = acos(ada);
+        g.beta[i] = be;
+        sb = sin(be);
+        f1 = (be > 1e-10) ? d3(am, bi) / sb : d3(bm, bi);
+        Ac = acos(clamp1(f1));
+        f2 = d3(am, ci);
+        double al = (f2 >= 0) ? Ac : -Ac;
+        g.alpha[i] = al;
+        f1 = clamp1(d6(bi, bm, ci, cf) / (1.0 + ada));
+        apg = acos(f1);
+        f2 = (d3(ci, bm) - d3(bi, cf)) / (1.0 + ada);
+        g.gamma[i] = (f2 >= 0.0) ? apg - al : -apg - al;
+    } else {
+        const int ic = s.ic_of[i];
+        double *ad = g.a_dna + 3 * ic, *bd = g.b_dna + 3 * ic, *cd = g.c_dna + 3 * ic;
+        const double len = g.length[i];
+        double adv[3];
+        for (int k = 0; k < 3; k++) adv[k] = (rf[k] - (ri[k] - kRO * bi[k] - kD1 * ci[k])) / len;
+        ad[0] = adv[0]; ad[1] = adv[1]; ad[2] = adv[2];
+        double cb = clamp1(d3(ai, adv));
+        double bp = acos(cb);
+        g.beta_p[ic] = bp;
+        sb = sin(bp);
+        double bdv[3], cdv[3];
+        if (bp >= 1e-10) {
+            double bm[3];
+            for (int k = 0; k < 3; k++) bm[k] = (adv[k] - cb * ai[k]) / sb;
+            double ca = clamp1(d3(bm, bi));
+            Ac = acos(ca);
+            f1 = d3(adv, ci);
+            double alp = (f1 >= 0) ? Ac : -Ac;
+            double gap = -alp;
+            g.alpha_p[ic] = alp;
+            g.gamma_p[ic] = gap;
+            double sa = sin(alp), sg = sin(gap), cg = cos(gap);
+            double R21 = -cg * sb, R22 = cg * cb * ca - sg * sa, R23 = cg * cb * sa + sg * ca;
+            double R31 = sg * sb, R32 = -sg * cb * ca - cg * sa, R33 = -sg * cb * sa + cg * ca;
+            for (int k = 0; k < 3; k++) {
+                bdv[k] = R21 * ai[k] + R22 * bi[k] + R23 * ci[k];
+                cdv[k] = R31 * ai[k] + R32 * bi[k] + R33 * ci[k];
+            }
+        } else {
+            for (int k = 0; k < 3; k++) { bdv[k] = bi[k]; cdv[k] = ci[k]; }
+        }
+        for (int k = 0; k < 3; k++) { bd[k] = bdv[k]; cd[k] = cdv[k]; }
+        ada = clamp1(d3(adv, af));
+        double be = acos(ada);
+        g.beta[i] = be;
+        sb = sin(be);
+        f1 = (be >= 1e-10) ? d3(af, bdv) / sb : d3(bf, bdv);
+        Ac = acos(clamp1(f1));
+        f2 = d3(af, cdv);
+        double al = (f2 >= 0) ? Ac : -Ac;
+        g.alpha[i] = al;
+        f1 = clamp1(d6(bdv, bf, cdv, cf) / (1.0 + ada));
+        apg = acos(f1);
+        f2 = (d3(cdv, bf) - d3(bdv, cf)) / (1.0 + ada);
+        g.gamma[i] = (f2 >= 0.0) ? apg - al : -apg - al;
+    }
+}
+
+// update_tail_beta (func.cpp:1763-1790) and update_LH_beta (2062-2089)
+__global__ void k_betas(Stat s, Geo g, int do_t, int do_l) {
+#pragma clang fp contract(off)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double *p1, *p2, *p3;
+    double *out;
+    bool on;
+    if (i < s.n_tail) {
+        if (!do_t) return;
+        on = (i < s.n_tail - 2) && s.t_grp[i] == s.t_grp[i + 1] && s.t_grp[i] == s.t_grp[i + 2];
+        p1 = g.r_t + 3 * i; p2 = p1 + 3; p3 = p1 + 6;
+        out = g.beta_t + i;
+    } else {
+        i -= s.n_tail;
+        if (i >= s.n_LH || !do_l) return;
+        on = (i % s.n_lh_c != 0) && (i % s.n_lh_c != s.n_lh_c - 1);
+        p1 = g.r_lh + 3 * i - 3; p2 = g.r_lh + 3 * i; p3 = g.r_lh + 3 * i + 3;
+        out = g.beta_lh + i;
+    }
+    if (!on) { *out = 0.0; return; }
+    double x12 = p1[0] - p2[0], y12 = p1[1] - p2[1], z12 = p1[2] - p2[2];
+    double r12 = sqrt(x12 * x12 + y12 * y12 + z12 * z12);
+    double x32 = p3[0] - p2[0], y32 = p3[1] - p2[1], z32 = p3[2] - p2[2];
+    double r32 = sqrt(x32 * x32 + y32 * y32 + z32 * z32);
+    *out = acos((x12 * x32 + y12 * y32 + z12 * z32) / (r12 * r32));
+}
+
+// rotation_cal + rotate (func_cuda.cu:681-704, func.cpp:1607-1670): one
+// thread per bead, the increment of every bead formed before any frame is
+// written (the reference GPU kernel races here, func_cuda.cu:702).
+__global__ void k_rotate(Stat s, Geo src, Geo dst, const double *T, const double *rr) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= s.n) return;
+    const double zc = 8 * kPI * kETA * 125.0, zl = 4 * kPI * kETA * kRH * kRH * kLO;
+    double wa, wb, wc;
+    if (s.type[j] != 0) {
+        wa = s.time_step * T[3 * j] / zc + rr[3 * j];
+        wb = s.time_step * T[3 * j + 1] / zc + rr[3 * j + 1];
+        wc = s.time_step * T[3 * j + 2] / zc + rr[3 * j + 2];
+    } else {
+        wa = s.time_step * T[3 * j] / zl + rr[3 * j];
+        wb = 0.0;
+        wc = 0.0;
+    }
+    const double *a = src.a + 3 * j, *b = src.b + 3 * j, *c = src.c + 3 * j;
+    double *an = dst.a + 3 * j, *bn = dst.b + 3 * j, *cn = dst.c + 3 * j;
+    double wa2 = wa * wa, wb2 = wb * wb, wc2 = wc * wc;
+    double z2 = wa2 + wb2 + wc2, z = sqrt(z2);
+    double czt = cos(z), szt = sin(z), om = 1.0 - czt;
+    if (!(z2 > 0.0)) {
+        for (int k = 0; k < 3; k++) { an[k] = a[k]; bn[k] = b[k]; cn[k] = c[k]; }
+        return;
+    }
+    double ga0 = ((wb2 + wc2) * czt + wa2) / z2, ga1 = wa * wb * om / z2 + wc * szt / z,
+           ga2 = wa * wc * om / z2 - wb * szt / z;
+    double gb0 = wa * wb * om / z2 - wc * szt / z, gb1 = ((wa2 + wc2) * czt + wb2) / z2,
+           gb2 = wb * wc * om / z2 + wa * szt / z;
+    double gc0 = wa * wc * om / z2 + wb * szt / z, gc1 = wb * wc * om / z2 - wa * szt / z,
+           gc2 = ((wa2 + wb2) * czt + wc2) / z2;
+    double A[3] = {a[0], a[1], a[2]}, B[3] = {b[0], b[1], b[2]}, C[3] = {c[0], c[1], c[2]};
+    for (int k = 0; k < 3; k++) {
+        an[k] = ga0 * A[k] + ga1 * B[k] + ga2 * C[k];
+        bn[k] = gb0 * A[k] + gb1 * B[k] + gb2 * C[k];
+        cn[k] = gc0 * A[k] + gc1 * B[k] + gc2 * C[k];
+    }
+}
+
+// RPY 3x3 block between beads at ri and rj (func.cpp:752-799); rows are
+// the components of bead i.  Identical value for (j, i): only products of
+// pairs of components and squares of r_i - r_j enter.
+__device__ __forceinline__ void rpy_block(double3 ri, double3 rj, double radi, double radj, double a1,
+                                          double a2, double M[3][3]) {
+    double sij[3] = {ri.x - rj.x, ri.y - rj.y, ri.z - rj.z};
+    double ssq = sij[0] * sij[0] + sij[1] * sij[1] + sij[2] * sij[2];
+    double s = sqrt(ssq);
+    if (s >= radi + radj) {
+        double f = (radi * radi + radj * radj) / ssq;
+        double f1 = 1.0 + f / 3, f2 = 1.0 - f;
+        double pre = a2 / s;
+        for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 3; l++)
+                M[k][l] = (k == l) ? pre * (f1 + f2 * sij[k] * sij[k] / ssq) : pre * (f2 * sij[k] * sij[l] / ssq);
+    } else {
+        double se = pow((radi * radi * radi + radj * radj * radj) / 2.0, 1.0 / 3);
+        double pre = a1 / se, sq = sqrt(ssq);
+        for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 3; l++)
+                M[k][l] = (k == l) ? pre * (1.0 - 9.0 * sq / (32 * se) + 3.0 * sij[k] * sij[k] / (32.0 * se * sq))
+                                   : pre * (3.0 * sij[k] * sij[l] / (32.0 * se * sq));
+    }
+}
+
+// translation_cal (func_cuda.cu:650-679) without a stored tensor: the drift
+// del * D_blk * F of bead i is summed over the beads of its species block
+// with the RPY blocks recomputed in closed form (same entries, same
+// summation order as a row of the stored D).  Beads outside the
+// hydrodynamic set (hi_all == 0: tails, LH) see their self-mobility only.
+__global__ void k_translate(Stat s, int hi_all, Geo src, Geo dst, Frc f, const double *rd,
+                            const double *rad_all) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = s.n, nt = s.n_tail, nl = s.n_LH;
+    if (b >= n + nt + nl) return;
+    const double *x0, *F;
+    double *out;
+    int lo, cnt;
+    if (b < n) {
+        x0 = src.r; F = f.F; out = dst.r; lo = 0; cnt = n;
+    } else if (b < n + nt) {
+        x0 = src.r_t; F = f.tF; out = dst.r_t; lo = n; cnt = nt;
+    } else {
+        x0 = src.r_lh; F = f.lF; out = dst.r_lh; lo = n + nt; cnt = nl;
+    }
+    const int i = b - lo;
+    double v[3];
+    for (int k = 0; k < 3; k++) v[k] = x0[3 * i + k] + rd[3 * b + k];
+    const double radi = rad_all[b];
+    if (b < n || hi_all) {
+        const double3 ri = ld3(x0 + 3 * i);
+        for (int j = 0; j < cnt; j++) {
+            const double *Fj = F + 3 * j;
+            if (j == i) {
+                double dself = s.a1 / radi;
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++) v[k] = v[k] + s.del * ((k == l) ? dself : 0.0) * Fj[l];
+                continue;
+            }
+            double M[3][3];
+            rpy_block(ri, ld3(x0 + 3 * j), radi, rad_all[lo + j], s.a1, s.a2, M);
+            for (int k = 0; k < 3; k++)
+                for (int l = 0; l < 3; l++) v[k] = v[k] + s.del * M[k][l] * Fj[l];
+        }
+    } else {
+        const double dself = s.a1 / radi;
+        for (int k = 0; k < 3; k++) v[k] = v[k] + s.del * dself * F[3 * i + k];
+    }
+    for (int k = 0; k < 3; k++) out[3 * i + k] = v[k];
+}
+
+// force_torque_tmp (func_cuda.cu:2626-2641)
+__global__ void k_average(int n3, int nt3, int nl3, Frc a, Frc b, Frc o) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) { o.F[i] = 0.5 * (a.F[i] + b.F[i]); o.T[i] = 0.5 * (a.T[i] + b.T[i]); }
+    if (i < nt3) o.tF[i] = 0.5 * (a.tF[i] + b.tF[i]);
+    if (i < nl3) o.lF[i] = 0.5 * (a.lF[i] + b.lF[i]);
+}
+
+// final_updates (func_cuda.cu:2643-2662)
+__global__ void k_commit(int n3, int nt3, int nl3, Geo from, Geo to) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) { to.r[i] = from.r[i]; to.a[i] = from.a[i]; to.b[i] = from.b[i]; to.c[i] = from.c[i]; }
+    if (i < nt3) to.r_t[i] = from.r_t[i];
+    if (i < nl3) to.r_lh[i] = from.r_lh[i];
+}
+
+void launch_frames(const Stat &s, const Geo &g, hipStream_t q) {
+    if (s.n < 2) return;
+    k_euler1<<<nblk(s.n - 1, 64), 64, 0, q>>>(s, g);
+    k_euler2<<<nblk(s.n - 1, 64), 64, 0, q>>>(s, g);
+}
+
+void launch_betas(const Stat &s, const Geo &g, int tails, int lh, hipStream_t q) {
+    long long tot = (long long)s.n_tail + s.n_LH;
+    if (tot > 0 && (tails || lh)) k_betas<<<nblk(tot, 128), 128, 0, q>>>(s, g, tails, lh);
+}
+
+void launch_rotate(const Stat &s, const Geo &src, const Geo &dst, const double *T, const double *rr,
+                   hipStream_t q) {
+    k_rotate<<<nblk(s.n, 64), 64, 0, q>>>(s, src, dst, T, rr);
+}
+
+void launch_translate(const Stat &s, int hi_all, const Geo &src, const Geo &dst, const Frc &f,
+                      const double *rd, const double *rad_all, hipStream_t q) {
+    long long tot = (long long)s.n + s.n_tail + s.n_LH;
+    k_translate<<<nblk(tot, 64), 64, 0, q>>>(s, hi_all, src, dst, f, rd, rad_all);
+}
+
+void launch_average(const Stat &s, const Frc &a, const Frc &b, const Frc &out, hipStream_t q) {
+    int mx = 3 * s.n;
+    if (3 * s.n_tail > mx) mx = 3 * s.n_tail;
+    if (3 * s.n_LH > mx) mx = 3 * s.n_LH;
+    k_average<<<nblk(mx, 256), 256, 0, q>>>(3 * s.n, 3 * s.n_tail, 3 * s.n_LH, a, b, out);
+}
+
+void launch_commit(const Stat &s, const Geo &from, const Geo &to, hipStream_t q) {
+    int mx = 3 * s.n;
+    if (3 * s.n_tail > mx) mx = 3 * s.n_tail;
+    if (3 * s.n_LH > mx) mx = 3 * s.n_LH;
+    k_commit<<<nblk(mx, 256), 256, 0, q>>>(3 * s.n, 3 * s.n_tail, 3 * s.n_LH, from, to);
+}
+
+}  // namespace bd

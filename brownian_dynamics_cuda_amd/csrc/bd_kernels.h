@@ -1,0 +1,56 @@
+// bd_kernels.h -- host-side launchers of the BD step kernels.
+#pragma once
+#include "bd_dev.h"
+
+namespace bd {
+
+// Scratch of the core-core charge sums.
+struct ForceScratch {
+    unsigned char *cc_valid;  // n_c * n_c
+    int *cc_list;             // 2 * n_c * (n_c - 1) / 2
+    int *cc_count;            // 1
+    double *cc_slots;         // n_c * n_c * nchunk * 10
+    int cc_nchunk;
+    int cc_grid;
+};
+
+// forces (bd_forces.hip)
+void launch_sites(const Stat &s, const Geo &g, const Sites &st, hipStream_t q);
+// mark(arg, id, phase): optional timing hook called around each kernel
+// (phase 0 before, 1 after); ids are the K_* values below.
+enum KernelId { K_SITES, K_MECH, K_LINKER, K_TAIL, K_LH, K_CORE, K_CCLIST, K_CCWORK, K_CCRED,
+                K_RPY, K_POTRF, K_TRSM, K_SYRK, K_NOISE, K_FRAMES, K_BETAS, K_ROTATE, K_TRANSLATE,
+                K_AVG, K_COMMIT, K_EREDUCE, K_COUNT };
+typedef void (*MarkFn)(void *, int, int);
+void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, const ForceScratch &w,
+                   double *Epart, int terms, hipStream_t q, MarkFn mark = nullptr, void *arg = nullptr);
+int energy_slots(const Stat &s);
+void launch_energy_reduce(const double *Epart, int ne, double *out, hipStream_t q);
+void launch_fill(double *p, long long n, double v, hipStream_t q);
+void launch_fill_int(int *p, long long n, int v, hipStream_t q);
+
+// frames and integrator (bd_integrate.hip)
+void launch_frames(const Stat &s, const Geo &g, hipStream_t q);         // update_Euler_Angle
+void launch_betas(const Stat &s, const Geo &g, int tails, int lh, hipStream_t q);
+void launch_rotate(const Stat &s, const Geo &src, const Geo &dst, const double *T, const double *rr,
+                   hipStream_t q);
+void launch_translate(const Stat &s, int hi_all, const Geo &src, const Geo &dst, const Frc &f,
+                      const double *rd, const double *rad_all, hipStream_t q);
+void launch_average(const Stat &s, const Frc &a, const Frc &b, const Frc &out, hipStream_t q);
+void launch_commit(const Stat &s, const Geo &from, const Geo &to, hipStream_t q);
+
+// diffusion tensor, Cholesky, noise (bd_chol.hip)
+struct CholWork {
+    double *A;       // mpad x ld, row-major; lower triangle holds D then L
+    int m, mpad, ld;
+    double *Linv;    // 64 x 64 inverse of the current diagonal block
+    int *neg;        // negative-pivot counter
+    int mode;        // BD_CHOL_*
+};
+void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, const CholWork &c,
+                hipStream_t q);
+void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark = nullptr, void *mark_arg = nullptr);
+void launch_noise(const Stat &s, int n_hi, int n_D3, const CholWork &c, const double *p,
+                  const double *rad_all, double *rd, hipStream_t q);
+
+}  // namespace bd

@@ -1,0 +1,200 @@
+/*
+ * bd_hip.h -- C-ABI of the MI355X Brownian-dynamics step (libbdhip.so).
+ *
+ * The step is the Ermak-McCammon predictor-corrector of the reference
+ * (BD_cuda/func_cuda.cu:3434-3680, main_cuda) with the CPU semantics of
+ * BD_cuda/func.cpp.  Plain pointers and sizes only; every call returns 0 on
+ * success or a negative BD_E_* code, with a message in bd_last_error().
+ *
+ * The reference's own entry points (func_cuda.h) are provided on top of this
+ * API by include/func_cuda.h; this header is the extended interface used by
+ * the parity harness, the benchmark and the Python host mirror.
+ */
+#ifndef BD_HIP_H
+#define BD_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BD_OK 0
+#define BD_E_ARG (-1)      /* invalid argument or size                      */
+#define BD_E_HIP (-2)      /* HIP runtime error                              */
+#define BD_E_STATE (-3)    /* call out of order (no state set, ...)          */
+#define BD_E_CAPACITY (-4) /* a size exceeds what the context was made for  */
+
+/* Static description of a fiber.  Sizes, per-bead constants and physics
+ * scalars; the same quantities the reference passes to
+ * cuda_application_init_data (func_cuda.h:48, func_cuda.cu:2676-2887).
+ * Arrays are host pointers in the reference layout (AoS xyz, x[3i+k]) and
+ * are copied at bd_create; the caller keeps ownership. */
+typedef struct bd_system {
+    int n, n_c;              /* main beads (cores + linker beads), cores        */
+    int n_tail, n_t;         /* tail beads in total, tail beads per core        */
+    int n_lh_n, n_lh_g, n_lh_c; /* LH template: N-term, globular head, C-term   */
+    int n_LH;                /* free LH beads bound now ((N+C) per bound core)  */
+    int Nq;                  /* point charges per core                          */
+    int groups_per_core;     /* tail groups per core; reference literal 10       */
+    int hi_all;              /* 1: D over every bead; 0: cores + linker beads    */
+    const int *type;         /* n: 1 core, 0 linker bead                         */
+    const double *phi_o;     /* n: intrinsic twist (func.cpp:412-431)           */
+    const double *rad;       /* n: hydrodynamic radius                          */
+    const double *core_pos;  /* 3 Nq: charge positions, nm, core body frame     */
+    const double *core_q;    /* Nq                                              */
+    const double *tail_pos;  /* 3 n_t: tail template, body frame                */
+    const int *tail_fix_tpl; /* n_t: 1 = bead anchored to its core              */
+    const double *h_t, *g_t, *lo_t, *beta_o_t, *t_q, *t_rad; /* n_tail           */
+    const int *t_grp, *t_fix;                                 /* n_tail           */
+    const double *LH_g_pos;  /* 3 n_lh_g: globular head, body frame             */
+    const int *LH_conn;      /* n_lh_n + n_lh_g + n_lh_c                         */
+    const double *LH_q, *LH_vdw_hh, *LH_vdw_hc, *LH_vdw_hl, *LH_vdw_ht;
+    const double *LH_kstr, *LH_kben, *LH_streq, *LH_betaeq;   /* same length     */
+    const int *nc_lh_flag;   /* n_c: 1 = linker histone bound                   */
+    const double *lh_rad;    /* n_LH                                            */
+    double h, g, s, debyell, debye, q_l, k_e, k_ex;
+    double sig_dd, sig_dc, sig_cc, sig_tt, sig_tl, sig_tc;
+    double a1, a2, s2dt, del, time_step;
+} bd_system;
+
+/* Cholesky arithmetic for the factor of D (the noise B = L). */
+#define BD_CHOL_FP64 0      /* everything FP64 (FP64 MFMA updates)              */
+#define BD_CHOL_FP32 1      /* trailing updates FP32 MFMA, FP32 accumulation    */
+#define BD_CHOL_BF16 2      /* trailing updates BF16 MFMA, FP32 accumulation    */
+
+typedef struct bd_options {
+    int device;             /* HIP device ordinal                               */
+    int chol_mode;          /* BD_CHOL_*                                        */
+    int use_graph;          /* 1: replay each step as one captured hipGraph      */
+    int lh_capacity;        /* free LH beads to reserve (>= n_LH; 0 = n_LH)     */
+} bd_options;
+
+typedef struct bd_context bd_context;
+
+/* Create a context on one GPU: copies the static data, allocates every
+ * device buffer once (the reference reallocates per step,
+ * func_cuda.cu:3436-3534).  Replaces cuda_application_init_D_Chol +
+ * cuda_application_init_data (func_cuda.h:37,48). */
+int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out);
+int bd_destroy(bd_context *ctx); /* free_all (func_cuda.h:52) */
+
+/* Upload positions and frames (host).  With rebuild != 0 the derived frame
+ * quantities are recomputed on the GPU, as main.cpp:164,223,283 do on the
+ * host before the first step (linker frames are re-orthonormalised).
+ * Any pointer may be NULL when its size is 0. */
+int bd_set_state(bd_context *ctx, const double *r, const double *a, const double *b,
+                 const double *c, const double *r_t, const double *r_lh, int rebuild);
+int bd_get_state(bd_context *ctx, double *r, double *a, double *b, double *c, double *r_t,
+                 double *r_lh);
+
+/* Rebind the linker histones: which cores carry one (nc_lh_flag[n_c]), the
+ * free-bead count n_LH = bound x (n_lh_n + n_lh_c) <= lh_capacity, their
+ * positions and radii.  This is the bookkeeping main_cuda redoes every
+ * step (func_cuda.cu:3436-3492) after the host's bind/unbind move
+ * (main.cpp:386-465); here it is a copy into reserved buffers. */
+int bd_set_lh(bd_context *ctx, const int *nc_lh_flag, int n_LH, const double *r_lh,
+              const double *lh_rad);
+
+/* Integrator scalars: RPY prefactors a1 = kT/(6 pi eta), a2 = kT/(8 pi eta),
+ * s2dt = sqrt(2 dt), del = dt/kT and dt (main.cpp:88-99); bd_create takes
+ * them from bd_system, the legacy main_cuda passes them every step. */
+int bd_set_integrator(bd_context *ctx, double a1, double a2, double s2dt, double del,
+                      double time_step);
+
+/* Derived per-bead quantities (Euler angles, segment lengths, core exit
+ * frames, tail and LH bending angles), in the layout of func.h:66. */
+typedef struct bd_derived {
+    double *alpha, *beta, *gamma, *length;     /* n    */
+    double *a_dna, *b_dna, *c_dna;             /* 3n_c */
+    double *alpha_p, *beta_p, *gamma_p;        /* n_c  */
+    double *beta_t;                            /* n_tail */
+    double *beta_lh;                           /* n_LH */
+} bd_derived;
+int bd_set_derived(bd_context *ctx, const bd_derived *d);
+int bd_get_derived(bd_context *ctx, bd_derived *d);
+
+/* One BD step with host noise: rr[3n] rotational, p[3(n+n_tail+n_LH)]
+ * translational (main.cpp:479-486 draw order), energy of the predictor
+ * pass (func_cuda.cu:3613-3615).  This is main_cuda without the legacy
+ * H2D/D2H traffic. */
+int bd_step(bd_context *ctx, const double *rr, const double *p, double *energy);
+
+/* Same with noise already in device memory; energy written to a device
+ * double (may be NULL).  No host synchronisation. */
+int bd_step_device(bd_context *ctx, const double *d_rr, const double *d_p, double *d_energy);
+
+/* Forces at the current state (one force pass, func_cuda.cu:3556-3613).
+ * terms: bit mask BD_TERM_*.  Outputs are host arrays (NULL to skip). */
+#define BD_TERM_MECH 1
+#define BD_TERM_ELEC 2
+#define BD_TERM_TAIL 4
+#define BD_TERM_LH 8
+#define BD_TERM_ALL 15
+int bd_forces(bd_context *ctx, int terms, double *F, double *T, double *tF, double *lhF,
+              double *energy);
+
+/* Diffusion tensor and its factor at the current state.  bd_factor builds D
+ * and factors it in place; bd_get_factor copies L (m x m row-major, lower,
+ * m = 3 * n_hi) and the count of negative pivots replaced by |f|
+ * (func.cpp:827-833).  bd_get_tensor assembles D only (no factor). */
+int bd_factor(bd_context *ctx);
+int bd_get_factor(bd_context *ctx, double *L, int *neg_pivots);
+int bd_get_tensor(bd_context *ctx, double *D);
+/* rd = s2dt * L * p for the current factor (rd_cal, func_cuda.cu:634-647). */
+int bd_noise_displacement(bd_context *ctx, const double *p, double *rd);
+int bd_set_chol_mode(bd_context *ctx, int chol_mode);
+
+/* Per-kernel timing with HIP events on the context's stream (0 = off). */
+int bd_set_timing(bd_context *ctx, int enable);
+typedef struct bd_kernel_time {
+    char name[48];
+    double total_ms;
+    long long launches;
+} bd_kernel_time;
+int bd_get_timing(bd_context *ctx, bd_kernel_time *out, int max_entries, int *n_entries);
+
+/* Counters of the last force pass: charge pairs evaluated (core-core),
+ * core pairs within 25 nm, and so on; used for algorithmic work figures. */
+typedef struct bd_work_counts {
+    long long core_pairs;          /* core-core pairs inside 25 nm                */
+    long long core_charge_pairs;   /* charge-charge evaluations for those pairs   */
+    long long core_linker_pairs;   /* core-linker pairs inside 25 nm              */
+} bd_work_counts;
+int bd_get_work_counts(bd_context *ctx, bd_work_counts *out);
+
+/* Sizes of the context. */
+int bd_get_sizes(bd_context *ctx, int *n, int *n_tail, int *n_LH, int *n_hi);
+
+/* Device memory helpers for callers without their own allocator. */
+int bd_device_alloc(size_t bytes, void **ptr);
+int bd_device_free(void *ptr);
+int bd_memcpy_htod(void *dst, const void *src, size_t bytes);
+int bd_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int bd_synchronize(bd_context *ctx);
+
+/* Host noise: MT19937 + Marsaglia polar normals with the cached second
+ * value (replaces mt.cpp:48-141 and rand_normal, utilities.cpp:304-336).
+ * bd_rng_step_noise draws one step in the order of main.cpp:386-388 and
+ * 479-486 (two uniforms every 100th step, rr[n3] with sigma r_var[i],
+ * then p[n_D3]); lh_uniforms (may be NULL) receives the two uniforms.
+ * The state (624 words, index, cached normal) can be saved and restored,
+ * which the reference cannot do (its seed is time(0), main.cpp:60-62). */
+typedef struct bd_rng bd_rng;
+int bd_rng_create(unsigned int seed, bd_rng **out);
+int bd_rng_destroy(bd_rng *g);
+unsigned int bd_rng_u32(bd_rng *g);
+double bd_rng_uniform(bd_rng *g);
+double bd_rng_normal(bd_rng *g, double mean, double stddev);
+int bd_rng_step_noise(bd_rng *g, int step, int n3, const double *r_var, int n_D3, double *rr,
+                      double *p, double *lh_uniforms);
+int bd_rng_get_state(bd_rng *g, unsigned int *words, int *idx, int *cached, double *n2);
+int bd_rng_set_state(bd_rng *g, const unsigned int *words, int idx, int cached, double n2);
+
+const char *bd_last_error(void);
+int bd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
