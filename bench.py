@@ -1,0 +1,215 @@
+"""Benchmark: BD steps/s of the 100-nucleosome fiber on one MI355X.
+
+One step = one Ermak-McCammon predictor-corrector step of the whole fiber
+(RPY tensor over cores + linker beads, its Cholesky factor, noise product,
+two force passes, rotation/translation, frame rebuild), as main_cuda does
+(func_cuda.cu:3434-3680).  Noise is drawn on the host in the reference
+order (MT19937 seed 5489) for every step before timing and uploaded to
+HBM, so the timed region reads inputs already resident on the GPU.
+
+Prints one JSON line (see the keys below).  --gpus N > 1 is refused: this
+build runs one fiber on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from brownian_dynamics_cuda_amd import engine as E  # noqa: E402
+from brownian_dynamics_cuda_amd import fiber as Fb  # noqa: E402
+
+PEAK_FP64 = 78.6e12  # MI355X FP64 (matrix and vector), FLOP/s
+PEAK_FP32_MFMA = 157.3e12
+PEAK_HBM = 8.0e12    # B/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--config", type=int, default=2, help="benchmark configuration (1-5)")
+    ap.add_argument("--timing-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="reference CPU steps to time (0: skip)")
+    ap.add_argument("--no-mixed", action="store_true")
+    return ap.parse_args()
+
+
+def noise_stream(fib, nsteps, seed=5489):
+    """(rr, p) for nsteps steps in the reference draw order."""
+    rng = E.HostRng(seed)
+    rv = fib.r_var()
+    n3 = 3 * fib.n
+    nD3 = 3 * (fib.n + fib.n_tail + fib.n_LH)
+    rr = np.zeros((nsteps, n3))
+    p = np.zeros((nsteps, nD3))
+    for s in range(nsteps):
+        rr[s], p[s] = rng.step_noise(s, rv, nD3)
+    return rr, p
+
+
+def chol_flops(m):
+    return m ** 3 / 3.0
+
+
+def syrk_flops_total(m, tb=64):
+    """FLOPs of the trailing updates of the tiled factor (2 per MAC)."""
+    nt = (m + tb - 1) // tb
+    tot = 0.0
+    for kb in range(nt - 1):
+        M = nt - kb - 1
+        tot += M * (M + 1) / 2 * 2.0 * tb ** 3
+    return tot
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs one fiber on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    cfg = args.config
+    fib, st = Fb.make_config(cfg)
+    n_hi3 = 3 * fib.n_hi
+    K, W = args.steps, args.warmup
+    t0 = time.time()
+    rr, p = noise_stream(fib, W + K)
+    t_noise = time.time() - t0
+    n3, nD3 = rr.shape[1], p.shape[1]
+
+    eng = E.BDEngine(fib, chol_mode=E.CHOL_FP64, use_graph=True)
+    eng.set_state(st, rebuild=True)
+    d_rr = E.DeviceBuffer(rr.nbytes)
+    d_p = E.DeviceBuffer(p.nbytes)
+    d_rr.upload(rr)
+    d_p.upload(p)
+    d_e = E.DeviceBuffer(8 * (W + K))
+
+    def run(s0, s1):
+        for s in range(s0, s1):
+            eng.step_device(d_rr.ptr + 8 * n3 * s, d_p.ptr + 8 * nD3 * s, d_e.ptr + 8 * s)
+
+    run(0, W)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    run(W, W + K)
+    eng.synchronize()
+    dt = time.perf_counter() - t0
+    energies = d_e.download(W + K)
+    final = eng.get_state()
+    finite = bool(np.all(np.isfinite(final.r)) and np.all(np.isfinite(energies)))
+    ms = 1e3 * dt / K
+    steps_per_s = K / dt
+
+    # per-kernel HIP-event timing on the step's stream (a few instrumented steps)
+    eng.set_timing(True)
+    for s in range(args.timing_steps):
+        eng.step(rr[s], p[s])
+    kt = eng.timing()
+    eng.set_timing(False)
+    counts = eng.work_counts()
+    per_step = {k: v[0] / args.timing_steps for k, v in kt.items()}
+    dominant = max(per_step, key=per_step.get)
+
+    # roofline of the dominant kernel
+    if dominant in ("syrk", "trsm", "potrf"):
+        flops = syrk_flops_total(n_hi3) if dominant == "syrk" else chol_flops(n_hi3)
+        tot_ms, launches = kt[dominant]
+        avg_s = tot_ms / launches / 1e3
+        per_launch = flops / (launches / args.timing_steps)
+        roof = dict(bound="mfma", achieved=per_launch / avg_s / 1e12, peak=PEAK_FP64 / 1e12,
+                    unit="TFLOP/s", traffic=None, kernel=dominant)
+    else:
+        # core-core charge sums: FLOPs counted per charge pair
+        pair_flops = 38.0
+        tot_ms, launches = kt[dominant]
+        avg_s = tot_ms / launches / 1e3
+        per_launch = counts["core_charge_pairs"] * pair_flops
+        roof = dict(bound="fp64", achieved=per_launch / avg_s / 1e12, peak=PEAK_FP64 / 1e12,
+                    unit="TFLOP/s", traffic=None, kernel=dominant)
+    roof["frac"] = roof["achieved"] / roof["peak"]
+
+    # mixed-precision factor: same state and noise, one step each way
+    mixed = None
+    if not args.no_mixed:
+        eng64 = E.BDEngine(fib, chol_mode=E.CHOL_FP64, use_graph=False)
+        eng32 = E.BDEngine(fib, chol_mode=E.CHOL_FP32, use_graph=True)
+        for e_ in (eng64, eng32):
+            e_.set_state(st, rebuild=True)
+        x0 = eng64.get_state()
+        eng64.step(rr[0], p[0])
+        eng32.step(rr[0], p[0])
+        a, b = eng64.get_state(), eng32.get_state()
+        disp64 = np.concatenate([a.r - x0.r, a.r_t - x0.r_t, a.r_lh - x0.r_lh])
+        disp32 = np.concatenate([b.r - x0.r, b.r_t - x0.r_t, b.r_lh - x0.r_lh])
+        rel = float(np.linalg.norm(disp32 - disp64) / np.linalg.norm(disp64))
+        eng32.set_state(st, rebuild=True)
+        d_e2 = E.DeviceBuffer(8 * (W + K))
+        for s in range(W):
+            eng32.step_device(d_rr.ptr + 8 * n3 * s, d_p.ptr + 8 * nD3 * s, d_e2.ptr + 8 * s)
+        eng32.synchronize()
+        t1 = time.perf_counter()
+        for s in range(W, W + K):
+            eng32.step_device(d_rr.ptr + 8 * n3 * s, d_p.ptr + 8 * nD3 * s, d_e2.ptr + 8 * s)
+        eng32.synchronize()
+        d32 = time.perf_counter() - t1
+        mixed = dict(chol="fp32-trailing", steps_per_s=K / d32, ms_per_step=1e3 * d32 / K,
+                     rel_displacement_err_vs_fp64=rel)
+        eng64.close()
+        eng32.close()
+
+    # CPU baseline: the reference's own func.cpp composed into one step
+    cpu = None
+    if args.cpu_steps > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        kind = "reference" if O.ref_available() else "port"
+        orc = O.Oracle(fib, "ref" if kind == "reference" else "or")
+        fs = orc.prepare(st)
+        t1 = time.perf_counter()
+        for s in range(args.cpu_steps):
+            orc.step(fs, rr[s], p[s])
+        dc = time.perf_counter() - t1
+        cpu = dict(value=args.cpu_steps / dc, unit="steps/s", cores=1, kind=kind,
+                   sample=f"{args.cpu_steps} steps of the same fiber, noise and start state "
+                          f"({dc:.1f} s), one thread")
+
+    out = {
+        "metric": "BD steps/s, 100-nucleosome fiber: FP64 vs mixed Cholesky at <=1e-6 rel err",
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded zigzag fiber, host MT19937 noise)",
+        "config": {"workload": f"cfg{cfg}: {fib.n_c}-nucleosome fiber, {fib.n} cores+linker beads, "
+                               f"{fib.n_tail} tail beads, {fib.n_LH} free LH beads, "
+                               f"D {n_hi3}x{n_hi3} FP64", "n_c": fib.n_c, "n_hi3": n_hi3},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "mixed": mixed,
+        "kernels_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items(), key=lambda x: -x[1])},
+        "work": counts,
+        "finite": finite,
+        "noise_gen_s": round(t_noise, 2),
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = steps_per_s / cpu["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -150,80 +150,116 @@ __device__ __forceinline__ void acc_to_lds(double *C, V acc[2][2], double sign, 
             }
 }
 
-// Unblocked right-looking Cholesky of the LDS tile L (in place, lower) and
-// the inverse X = L^-1 by forward substitution, one column step at a time.
-// f < 0 at a pivot is replaced by -f and counted (func.cpp:827-833).
-__device__ void potrf_inv_lds(double *L, double *X, int *neg) {
-    const int tid = threadIdx.x;
-    for (int e = tid; e < TB * TB; e += blockDim.x) {
-        int r = e / TB, c = e % TB;
-        X[r * LDP + c] = (r == c) ? 1.0 : 0.0;
-    }
-    __syncthreads();
-    for (int j = 0; j < TB; j++) {
-        if (tid == 0) {
-            double f = L[j * LDP + j];
-            if (f < 0) { f = -f; atomicAdd(neg, 1); }
-            L[j * LDP + j] = sqrt(f);
-        }
-        __syncthreads();
-        const double d = L[j * LDP + j];
-        if (tid < TB) {
-            if (tid > j) L[tid * LDP + j] = L[tid * LDP + j] / d;
-        } else if (tid < 2 * TB) {
-            int c = tid - TB;
-            if (c <= j) X[j * LDP + c] = X[j * LDP + c] / d;
-        }
-        __syncthreads();
-        for (int e = tid; e < TB * TB; e += blockDim.x) {
-            int t = e / TB, k = e % TB;
-            if (t <= j) continue;
-            double ltj = L[t * LDP + j];
-            if (k > j && k <= t) L[t * LDP + k] -= ltj * L[k * LDP + j];
-            else if (k <= j) X[t * LDP + k] -= ltj * X[j * LDP + k];
-        }
-        __syncthreads();
-    }
+// 1/sqrt(f) to full FP64 accuracy: v_rsq_f64 refined by two Newton steps.
+__device__ __forceinline__ double rsqrt_f64(double f) {
+    double y = __builtin_amdgcn_rsq(f);
+    y = y * (1.5 - 0.5 * f * y * y);
+    y = y * (1.5 - 0.5 * f * y * y);
+    return y;
 }
 
-__device__ __forceinline__ void store_inv(double *Linv, const double *X) {
-    for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
-        int r = e / TB, c = e % TB;
-        Linv[e] = X[r * LDP + c];
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Cholesky of the 64x64 LDS tile L (lower, in place), the CPU algorithm of
+// func.cpp:806-836 reordered into 16-column sub-panels.  A pivot f < 0 is
+// replaced by -f and counted (func.cpp:827-833); the pivot's inverse is
+// kept in invd[] for the triangular solves.  Each sub-panel is factored by
+// wavefront 0 alone -- lane r holds row r, the 16 columns in registers,
+// pivots and column entries are broadcast with readlane, so a column step
+// has no barrier -- then all waves apply its rank-16 update to the rest of
+// the tile.  Requires blockDim.x == 256.
+__device__ void potrf_tile_lds(double *L, double *invd, int *neg) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int sp = 0; sp < 4; sp++) {
+        const int c0 = 16 * sp;
+        if (w == 0) {
+            double l[16];
+            const bool act = lane >= c0;
+#pragma unroll
+            for (int k = 0; k < 16; k++) l[k] = act ? L[lane * LDP + c0 + k] : 0.0;
+#pragma unroll
+            for (int jj = 0; jj < 16; jj++) {
+                const int j = c0 + jj;
+                double f = readlane_f64(l[jj], j);
+                if (f < 0) {
+                    f = -f;
+                    if (lane == 0) atomicAdd(neg, 1);
+                }
+                const double inv = rsqrt_f64(f);
+                if (lane == j) { l[jj] = f * inv; invd[j] = inv; }
+                else if (lane > j) l[jj] = l[jj] * inv;
+#pragma unroll
+                for (int kk = jj + 1; kk < 16; kk++) {
+                    const double ckj = readlane_f64(l[jj], c0 + kk);  // L[c0+kk][j]
+                    if (lane >= c0 + kk) l[kk] -= l[jj] * ckj;
+                }
+            }
+            if (act)
+#pragma unroll
+                for (int k = 0; k < 16; k++) L[lane * LDP + c0 + k] = l[k];
+        }
+        __syncthreads();
+        // rank-16 update of the tile below and right of the sub-panel
+        const int base = c0 + 16;
+        const int nr = TB - base;
+        for (int e = tid; e < nr * nr; e += blockDim.x) {
+            const int r = base + e / nr, k = base + e % nr;
+            if (k > r) continue;
+            double acc = L[r * LDP + k];
+#pragma unroll
+            for (int c = 0; c < 16; c++) acc -= L[r * LDP + c0 + c] * L[k * LDP + c0 + c];
+            L[r * LDP + k] = acc;
+        }
+        __syncthreads();
     }
 }
 
 // first diagonal tile
-__global__ void __launch_bounds__(256) k_potrf0(double *A, int ld, double *Linv, int *neg) {
-    __shared__ double L[TB * LDP], X[TB * LDP];
+__global__ void __launch_bounds__(256) k_potrf0(double *A, int ld, double *invd, int *neg) {
+    __shared__ double L[TB * LDP];
     load_tile(A, ld, 0, 0, L);
     __syncthreads();
-    potrf_inv_lds(L, X, neg);
+    potrf_tile_lds(L, invd, neg);
     store_tile(A, ld, 0, 0, L, true);
-    store_inv(Linv, X);
 }
 
-// panel: L_ib,kb = A_ib,kb * Linv^T for ib > kb
-__global__ void __launch_bounds__(256) k_trsm(double *A, int ld, int kb, const double *Linv) {
-    __shared__ double As[TB * LDP], Bs[TB * LDP];
-    const int ib = kb + 1 + blockIdx.x;
-    load_tile(A, ld, ib * TB, kb * TB, As);
-    for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) Bs[(e / TB) * LDP + e % TB] = Linv[e];
+// Panel below the diagonal: X * L_kk^T = A_ib,kb for every row block
+// ib > kb, by forward substitution.  Lane = row (64 rows per wavefront,
+// four wavefronts per workgroup); the row lives in registers; L_kk and
+// the pivot inverses are broadcast from LDS.
+__global__ void __launch_bounds__(256) k_trsm(double *A, int ld, int kb, const double *invd, int nrow_blocks) {
+    __shared__ double Lk[TB * TB], iv[TB];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int e = tid; e < TB * TB; e += blockDim.x) {
+        const int r = e / TB, c = e % TB;
+        Lk[e] = (c <= r) ? A[(size_t)(kb * TB + r) * ld + kb * TB + c] : 0.0;
+    }
+    if (tid < TB) iv[tid] = invd[kb * TB + tid];
     __syncthreads();
-    dbl4 acc[2][2];
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-    mma_f64(As, Bs, acc);
-    __syncthreads();
-    acc_to_lds(As, acc, 1.0, true);
-    __syncthreads();
-    store_tile(A, ld, ib * TB, kb * TB, As, false);
+    const int ib = kb + 1 + blockIdx.x * 4 + w;
+    if (blockIdx.x * 4 + w >= nrow_blocks) return;
+    double *row = A + (size_t)(ib * TB + lane) * ld + kb * TB;
+    double b[TB];
+#pragma unroll
+    for (int k = 0; k < TB; k++) b[k] = row[k];
+#pragma unroll
+    for (int j = 0; j < TB; j++) {
+        b[j] = b[j] * iv[j];
+#pragma unroll
+        for (int k = j + 1; k < TB; k++) b[k] -= Lk[k * TB + j] * b[j];
+    }
+#pragma unroll
+    for (int k = 0; k < TB; k++) row[k] = b[k];
 }
 
 // trailing update of tile (ib, jb), kb < jb <= ib; the tile (kb+1, kb+1) is
-// then factored in place.  FP32 variant when mode != 0.
+// then factored in place.  FP32 MFMA variant when MODE != 0.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_syrk(double *A, int ld, int kb, double *Linv, int *neg) {
+__global__ void __launch_bounds__(256) k_syrk(double *A, int ld, int kb, double *invd, int *neg) {
     __shared__ double As[TB * LDP], Bs[TB * LDP];
     int ii, jj;
     tri_index(blockIdx.x, ii, jj);
@@ -252,9 +288,8 @@ __global__ void __launch_bounds__(256) k_syrk(double *A, int ld, int kb, double 
     }
     __syncthreads();
     if (ii == 0 && jj == 0) {
-        potrf_inv_lds(As, Bs, neg);
+        potrf_tile_lds(As, invd + ib * TB, neg);
         store_tile(A, ld, ib * TB, jb * TB, As, true);
-        store_inv(Linv, Bs);
     } else {
         store_tile(A, ld, ib * TB, jb * TB, As, ib == jb);
     }
@@ -296,7 +331,7 @@ void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_a
     for (int kb = 0; kb + 1 < nt; kb++) {
         const int M = nt - kb - 1;
         M_(K_TRSM, 0);
-        k_trsm<<<M, 256, 0, q>>>(c.A, c.ld, kb, c.Linv);
+        k_trsm<<<(M + 3) / 4, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, M);
         M_(K_TRSM, 1);
         M_(K_SYRK, 0);
         if (c.mode == 0)

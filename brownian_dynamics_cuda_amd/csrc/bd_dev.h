@@ -222,4 +222,27 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
     return t;  // valid on thread 0
 }
 
+// Block-wide sums of K doubles at once (fixed order); results valid on
+// thread 0.  sh must hold K * (NT / 64) doubles.
+template <int NT, int K>
+__device__ __forceinline__ void block_sum_n(double (&v)[K], double *sh) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < K; q++)
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < K; q++) sh[q * (NT / 64) + w] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            double t = 0.0;
+            for (int i = 0; i < NT / 64; i++) t += sh[q * (NT / 64) + i];
+            v[q] = t;
+        }
+    __syncthreads();
+}
+
 }  // namespace bd

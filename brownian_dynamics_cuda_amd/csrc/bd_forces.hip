@@ -216,35 +216,39 @@ __device__ __forceinline__ double ev_core(double kex, double sg, double d, doubl
 }
 
 // ---------------------------------------------------------------------------
-// Thread per linker bead: DNA-DNA, DNA-core (linker side), tail-linker,
-// LH-DNA and LHG-DNA (linker side).  F[j] +=.
-// Energy owner: DNA-DNA pairs with partner index > j.
+// Receiver-side kernels for linker, tail and LH beads: one workgroup per
+// receiving bead, its threads split the partner lists, and a fixed-order
+// block reduction forms the bead's force (deterministic, no atomics).
 // ---------------------------------------------------------------------------
-__global__ void k_linker(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
-    int li = blockIdx.x * blockDim.x + threadIdx.x;
-    if (li >= s.n_lnk) return;
+constexpr int kRecvThreads = 256;
+
+// Linker bead (func.cpp:1304-1487 linker side, 1898-1917, 2249-2279,
+// 2481-2522): DNA-DNA, DNA-core charges, tail-linker, LH-DNA, LHG-DNA.
+// F[j] +=.  Energy owner: DNA-DNA pairs with partner index > j.
+__global__ void __launch_bounds__(kRecvThreads) k_linker(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
+    __shared__ double red[4 * (kRecvThreads / 64)];
+    const int li = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
     const int j = s.lnk[li];
     const double3 rj = ld3(g.r + 3 * j);
     double3 F = zero3();
     double E = 0.0, Ed = 0.0;
     if ((terms & TERM_ELEC) && s.elig[j]) {
         const double kll = s.debyell, qlql = s.q_l * s.q_l;
-        for (int pi = 0; pi < s.n_lnk; pi++) {
+        for (int pi = tid; pi < s.n_lnk; pi += nth) {
             int i = s.lnk[pi];
             if (!s.elig[i] || abs(i - j) <= 1) continue;
             double3 dv = rj - ld3(g.r + 3 * i);
             double dist = sqrt(dot(dv, dv));
             if (dist > 25.0) continue;
-            double mi = 1.0 / dist;
             double Ep = 0.0;
             double g1 = s.k_e * qlql * exp(-kll * dist) * (kll * dist + 1.0) / (dist * dist);
             Ep += (s.k_e * qlql / dist) * exp(-kll * dist);
             if (dist <= 8) g1 += ev_core(s.k_ex, s.sig_dd, dist, Ep);
-            F += (g1 * mi) * dv;
+            F += (g1 / dist) * dv;
             if (i > j) E += Ep;
         }
-        // DNA-core: force on the linker from every charge of eligible cores
-        // within 25 nm (energy owned by the core side)
+        // DNA-core: every charge of the eligible cores within 25 nm
+        // (the core test is uniform across the workgroup); energy on the core side
         for (int k = 0; k < s.n_c; k++) {
             int cb = s.core_bead[k];
             if (!s.elig[cb]) continue;
@@ -252,7 +256,7 @@ __global__ void k_linker(Stat s, Geo g, Sites st, Frc f, double *Epart, int term
             if (sqrt(dot(dc, dc)) > 25.0) continue;
             const bool dh = abs(cb - j) > 1;
             const double *Q = st.Q + 3 * (size_t)k * s.Nq;
-            for (int q = 0; q < s.Nq; q++) {
+            for (int q = tid; q < s.Nq; q += nth) {
                 double3 z = rj - ld3(Q + 3 * q);
                 double d = sqrt(dot(z, z));
                 double cq = s.cq[q];
@@ -263,129 +267,143 @@ __global__ void k_linker(Stat s, Geo g, Sites st, Frc f, double *Epart, int term
             }
         }
     }
-    if (terms & TERM_TAIL) {
-        // tail-linker, linker side (func.cpp:1898-1917); energy owned by tails
-        for (int t = 0; t < s.n_tail; t++)
+    if (terms & TERM_TAIL)  // tail-linker, energy owned by tails
+        for (int t = tid; t < s.n_tail; t += nth)
             dhev_recv(s.t_q[t] * s.q_l, s.c4pe, s.debye, s.k_ex, s.sig_tl, rj, ld3(g.r_t + 3 * t), F, Ed);
-    }
     if (terms & TERM_LH) {
-        // LH-DNA (func.cpp:2249-2279), energy owned by LH beads
-        for (int i = 0; i < s.n_LH; i++) {
+        for (int i = tid; i < s.n_LH; i += nth) {  // LH-DNA, energy owned by LH beads
             int idx = i % s.nlh;
             idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
             dhev_recv(s.LH_q[idx] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[idx], rj, ld3(g.r_lh + 3 * i), F, Ed);
         }
-        // LHG-DNA (func.cpp:2481-2522), energy owned by the core side
-        for (int k = 0; k < s.n_lhc; k++)
-            for (int m = 0; m < s.n_lh_g; m++)
-                dhev_recv(s.LH_q[m] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[m], rj,
-                          ld3(st.G + 3 * (k * s.n_lh_g + m)), F, Ed);
+        for (int x = tid; x < s.n_lhc * s.n_lh_g; x += nth) {  // LHG-DNA, energy on the core side
+            int m = x % s.n_lh_g;
+            dhev_recv(s.LH_q[m] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[m], rj, ld3(st.G + 3 * x), F, Ed);
+        }
     }
-    double *Fj = f.F + 3 * j;
-    Fj[0] += F.x; Fj[1] += F.y; Fj[2] += F.z;
-    if (Epart) Epart[li] = E;
+    double v[4] = {F.x, F.y, F.z, E};
+    block_sum_n<kRecvThreads, 4>(v, red);
+    if (tid == 0) {
+        double *Fj = f.F + 3 * j;
+        Fj[0] += v[0]; Fj[1] += v[1]; Fj[2] += v[2];
+        if (Epart) Epart[li] = v[3];
+    }
 }
 
-// ---------------------------------------------------------------------------
-// Thread per tail bead (func.cpp:1793-2026 and the tail side of the LH
-// terms).  Assigns tF[j].  Energy owner: bonds / angles starting at j,
-// tail-tail pairs with partner > j, tail-linker pairs, the anchor spring.
-// ---------------------------------------------------------------------------
-__global__ void k_tail(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= s.n_tail) return;
+// Tail bead (func.cpp:1793-2026 and the tail side of 2343-2476).  Assigns
+// tF[j].  Energy owner: bonds / angles starting at j, tail-tail pairs with
+// partner > j, tail-linker pairs, the anchor spring.
+__global__ void __launch_bounds__(kRecvThreads) k_tail(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
+    __shared__ double red[4 * (kRecvThreads / 64)];
+    const int j = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
     const int nt = s.n_tail;
     const int *tg = s.t_grp;
     const double3 rj = ld3(g.r_t + 3 * j);
     double3 F = zero3();
     double E = 0.0, Ed = 0.0;
     if (terms & TERM_TAIL) {
-        // stretching of bonds (j-1,j) and (j,j+1)
-        if (j >= 1 && tg[j - 1] == tg[j]) {
-            double3 f1 = stretch_f1(s.h_t[j - 1], ld3(g.r_t + 3 * j - 3), rj, s.lo_t[j - 1], Ed);
-            F -= f1;
+        if (tid == 0) {
+            // stretching of bonds (j-1,j) and (j,j+1) (1811-1827)
+            if (j >= 1 && tg[j - 1] == tg[j])
+                F -= stretch_f1(s.h_t[j - 1], ld3(g.r_t + 3 * j - 3), rj, s.lo_t[j - 1], Ed);
+            if (j + 1 < nt && tg[j] == tg[j + 1]) F += stretch_f1(s.h_t[j], rj, ld3(g.r_t + 3 * j + 3), s.lo_t[j], E);
+            // bending of triples starting at j-2, j-1, j (1832-1858), subtracted
+            for (int c0 = j - 2; c0 <= j; c0++) {
+                if (c0 < 0 || c0 + 2 >= nt) continue;
+                if (!(tg[c0] == tg[c0 + 1] && tg[c0] == tg[c0 + 2])) continue;
+                double3 f1, f2, f3;
+                bend3(s.g_t[c0], g.beta_t[c0], s.beta_o_t[c0], ld3(g.r_t + 3 * c0), ld3(g.r_t + 3 * c0 + 3),
+                      ld3(g.r_t + 3 * c0 + 6), f1, f2, f3, (c0 == j) ? E : Ed);
+                F -= (c0 == j) ? f1 : (c0 == j - 1 ? f2 : f3);
+            }
+            // anchor spring to the owning core (1971-2022)
+            if (s.t_fix[j] == 1) {
+                int cb = s.core_bead[j / s.n_t];
+                double3 ro = body_to_world(ld3(g.r + 3 * cb), ld3(g.a + 3 * cb), ld3(g.b + 3 * cb),
+                                           ld3(g.c + 3 * cb), ld3(s.tail_pos + 3 * (j % s.n_t)));
+                double3 S = rj - ro;
+                E += s.h * (S.x * S.x + S.y * S.y + S.z * S.z) / 2;
+                F -= make_double3(s.h * S.x, s.h * S.y, s.h * S.z);
+            }
         }
-        if (j + 1 < nt && tg[j] == tg[j + 1]) F += stretch_f1(s.h_t[j], rj, ld3(g.r_t + 3 * j + 3), s.lo_t[j], E);
-        // bending of triples starting at j-2, j-1, j (forces subtracted)
-        for (int c0 = j - 2; c0 <= j; c0++) {
-            if (c0 < 0 || c0 + 2 >= nt) continue;
-            if (!(tg[c0] == tg[c0 + 1] && tg[c0] == tg[c0 + 2])) continue;
-            double3 f1, f2, f3;
-            bend3(s.g_t[c0], g.beta_t[c0], s.beta_o_t[c0], ld3(g.r_t + 3 * c0), ld3(g.r_t + 3 * c0 + 3),
-                  ld3(g.r_t + 3 * c0 + 6), f1, f2, f3, (c0 == j) ? E : Ed);
-            F -= (c0 == j) ? f1 : (c0 == j - 1 ? f2 : f3);
-        }
-        // tail-tail DH (func.cpp:1863-1891), skipping same-group neighbours
+        // tail-tail DH (1863-1891), skipping same-group neighbours
         const double qj = s.t_q[j];
-        for (int k = 0; k < nt; k++) {
+        for (int k = tid; k < nt; k += nth) {
             if (k == j) continue;
             if (!(tg[k] != tg[j] || abs(k - j) > 2)) continue;
             dh_recv(qj * s.t_q[k], s.c4pe, s.debye, rj, ld3(g.r_t + 3 * k), F, (k > j) ? E : Ed);
         }
-        // tail-linker (func.cpp:1898-1917)
-        for (int pi = 0; pi < s.n_lnk; pi++)
+        // tail-linker (1898-1917)
+        for (int pi = tid; pi < s.n_lnk; pi += nth)
             dhev_recv(qj * s.q_l, s.c4pe, s.debye, s.k_ex, s.sig_tl, rj, ld3(g.r + 3 * s.lnk[pi]), F, E);
-        // anchor spring to the owning core (func.cpp:1971-2022)
-        if (s.t_fix[j] == 1) {
-            int cb = s.core_bead[j / s.n_t];
-            double3 ro = body_to_world(ld3(g.r + 3 * cb), ld3(g.a + 3 * cb), ld3(g.b + 3 * cb),
-                                       ld3(g.c + 3 * cb), ld3(s.tail_pos + 3 * (j % s.n_t)));
-            double3 S = rj - ro;
-            E += s.h * (S.x * S.x + S.y * S.y + S.z * S.z) / 2;
-            F -= make_double3(s.h * S.x, s.h * S.y, s.h * S.z);
-        }
     }
     if ((terms & TERM_LH) && s.t_fix[j] == 0) {
-        // LH-tail (func.cpp:2343-2398) and LHG-tail (2403-2476), tail side;
-        // anchored tails pass these forces to a core (k_core)
-        for (int i = 0; i < s.n_LH; i++) {
+        // LH-tail and LHG-tail, tail side; anchored tails pass these to a core (k_core)
+        for (int i = tid; i < s.n_LH; i += nth) {
             int idx = i % s.nlh;
             idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
             dhev_recv(s.LH_q[idx] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rj, ld3(g.r_lh + 3 * i), F, Ed);
         }
-        for (int k = 0; k < s.n_lhc; k++)
-            for (int m = 0; m < s.n_lh_g; m++)
-                dhev_recv(s.LH_q[m] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], rj,
-                          ld3(st.G + 3 * (k * s.n_lh_g + m)), F, Ed);
+        for (int x = tid; x < s.n_lhc * s.n_lh_g; x += nth) {
+            int m = x % s.n_lh_g;
+            dhev_recv(s.LH_q[m] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], rj, ld3(st.G + 3 * x), F, Ed);
+        }
     }
-    st3(f.tF + 3 * j, F);
-    if (Epart) Epart[j] = E;
+    double v[4] = {F.x, F.y, F.z, E};
+    block_sum_n<kRecvThreads, 4>(v, red);
+    if (tid == 0) {
+        st3(f.tF + 3 * j, make_double3(v[0], v[1], v[2]));
+        if (Epart) Epart[j] = v[3];
+    }
 }
 
-// ---------------------------------------------------------------------------
-// Thread per free LH bead (func.cpp:2093-2781, LH side).  Assigns lF[j].
-// Energy owner: bonds / angles at j, LH-LH pairs with partner > j, LH-DNA,
-// LH-tail.
-// ---------------------------------------------------------------------------
-__global__ void k_lh(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= s.n_LH) return;
+// Free LH bead (func.cpp:2093-2781, LH side).  Assigns lF[j].  Energy
+// owner: bonds / angles at j, LH-LH pairs with partner > j, LH-DNA, LH-tail.
+__global__ void __launch_bounds__(kRecvThreads) k_lh(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
+    __shared__ double red[4 * (kRecvThreads / 64)];
+    const int j = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
     const int nlh = s.nlh;
     auto tpl = [&](int i) { int x = i % nlh; return (x < s.n_lh_n) ? x : x + s.n_lh_g; };
     const int idx = tpl(j);
+    const int mine = j / nlh;
     const double3 rj = ld3(g.r_lh + 3 * j);
     double3 F = zero3();
     double E = 0.0, Ed = 0.0;
     if (terms & TERM_LH) {
-        // stretching (func.cpp:2111-2143)
-        if (j >= 1 && s.LH_conn[tpl(j - 1)] == 1) {
-            int ip = tpl(j - 1);
-            F -= stretch_f1(s.LH_kstr[ip], ld3(g.r_lh + 3 * j - 3), rj, s.LH_streq[ip], Ed);
+        if (tid == 0) {
+            // stretching (2111-2143)
+            if (j >= 1 && s.LH_conn[tpl(j - 1)] == 1) {
+                int ip = tpl(j - 1);
+                F -= stretch_f1(s.LH_kstr[ip], ld3(g.r_lh + 3 * j - 3), rj, s.LH_streq[ip], Ed);
+            }
+            if (j + 1 < s.n_LH && s.LH_conn[idx] == 1)
+                F += stretch_f1(s.LH_kstr[idx], rj, ld3(g.r_lh + 3 * j + 3), s.LH_streq[idx], E);
+            // bending centred at j-1, j, j+1 (2149-2181)
+            for (int c0 = j - 1; c0 <= j + 1; c0++) {
+                if (c0 < 1 || c0 + 1 >= s.n_LH) continue;
+                if (!((c0 % s.n_lh_c != 0) && (c0 % s.n_lh_c != s.n_lh_c - 1))) continue;
+                int ic = tpl(c0);
+                double3 f1, f2, f3;
+                bend3(s.LH_kben[ic], g.beta_lh[c0], s.LH_betaeq[ic], ld3(g.r_lh + 3 * c0 - 3),
+                      ld3(g.r_lh + 3 * c0), ld3(g.r_lh + 3 * c0 + 3), f1, f2, f3, (c0 == j) ? E : Ed);
+                F -= (c0 == j + 1) ? f1 : (c0 == j ? f2 : f3);
+            }
+            // globular head - first C-term bead spring: force x10 (2727-2777)
+            if (j % nlh == 0) {
+                for (int m = 0; m < s.n_lh_g; m++) {
+                    int ti = m + s.n_lh_n;
+                    if (s.LH_conn[ti] != 1) continue;
+                    double3 G = ld3(st.G + 3 * (mine * s.n_lh_g + m));
+                    double3 dv = rj - G;
+                    double d = sqrt(dot(dv, dv));
+                    double ks = s.LH_kstr[ti], l0 = s.LH_streq[ti];
+                    double3 S = make_double3((d - l0) * dv.x / d, (d - l0) * dv.y / d, (d - l0) * dv.z / d);
+                    F -= make_double3(ks * S.x * 10, ks * S.y * 10, ks * S.z * 10);
+                }
+            }
         }
-        if (j + 1 < s.n_LH && s.LH_conn[idx] == 1) F += stretch_f1(s.LH_kstr[idx], rj, ld3(g.r_lh + 3 * j + 3), s.LH_streq[idx], E);
-        // bending centred at j-1, j, j+1 (func.cpp:2149-2181)
-        for (int c0 = j - 1; c0 <= j + 1; c0++) {
-            if (c0 < 1 || c0 + 1 >= s.n_LH) continue;
-            if (!((c0 % s.n_lh_c != 0) && (c0 % s.n_lh_c != s.n_lh_c - 1))) continue;
-            int ic = tpl(c0);
-            double3 f1, f2, f3;
-            bend3(s.LH_kben[ic], g.beta_lh[c0], s.LH_betaeq[ic], ld3(g.r_lh + 3 * c0 - 3),
-                  ld3(g.r_lh + 3 * c0), ld3(g.r_lh + 3 * c0 + 3), f1, f2, f3, (c0 == j) ? E : Ed);
-            F -= (c0 == j + 1) ? f1 : (c0 == j ? f2 : f3);
-        }
-        // LH-LH (func.cpp:2190-2244)
-        const int mine = j / nlh;
-        for (int k = 0; k < s.n_LH; k++) {
+        // LH-LH (2190-2244)
+        for (int k = tid; k < s.n_LH; k += nth) {
             if (k == j) continue;
             int i2 = tpl(k);
             double sg = (s.LH_hh[idx] + s.LH_hh[i2]) / 2;
@@ -396,35 +414,26 @@ __global__ void k_lh(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
                 dhev_recv(s.LH_q[idx] * s.LH_q[i2], s.c4pe, s.debye, s.k_ex, sg, rj, ld3(g.r_lh + 3 * k), F, Ek);
             }
         }
-        // LH-DNA (func.cpp:2249-2279)
-        for (int pi = 0; pi < s.n_lnk; pi++)
+        // LH-DNA (2249-2279)
+        for (int pi = tid; pi < s.n_lnk; pi += nth)
             dhev_recv(s.LH_q[idx] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[idx], rj, ld3(g.r + 3 * s.lnk[pi]), F, E);
-        // LH-tail (func.cpp:2343-2398)
-        for (int t = 0; t < s.n_tail; t++)
+        // LH-tail (2343-2398)
+        for (int t = tid; t < s.n_tail; t += nth)
             dhev_recv(s.LH_q[idx] * s.t_q[t], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rj, ld3(g.r_t + 3 * t), F, E);
-        // LHG of other cores - this bead (func.cpp:2590-2644), energy on the core side
-        for (int k = 0; k < s.n_lhc; k++) {
+        // LHG of other cores acting on this bead (2590-2644), energy on the core side
+        for (int x = tid; x < s.n_lhc * s.n_lh_g; x += nth) {
+            int k = x / s.n_lh_g, m = x % s.n_lh_g;
             if (k == mine) continue;
-            for (int m = 0; m < s.n_lh_g; m++)
-                dhev_recv(s.LH_q[m] * s.LH_q[idx], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[idx]) / 2, rj,
-                          ld3(st.G + 3 * (k * s.n_lh_g + m)), F, Ed);
-        }
-        // globular head - first C-term bead spring: force x10 (func.cpp:2727-2777)
-        if (j % nlh == 0) {
-            for (int m = 0; m < s.n_lh_g; m++) {
-                int ti = m + s.n_lh_n;
-                if (s.LH_conn[ti] != 1) continue;
-                double3 G = ld3(st.G + 3 * (mine * s.n_lh_g + m));
-                double3 dv = rj - G;
-                double d = sqrt(dot(dv, dv));
-                double ks = s.LH_kstr[ti], l0 = s.LH_streq[ti];
-                double3 S = make_double3((d - l0) * dv.x / d, (d - l0) * dv.y / d, (d - l0) * dv.z / d);
-                F -= make_double3(ks * S.x * 10, ks * S.y * 10, ks * S.z * 10);
-            }
+            dhev_recv(s.LH_q[m] * s.LH_q[idx], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[idx]) / 2, rj,
+                      ld3(st.G + 3 * x), F, Ed);
         }
     }
-    st3(f.lF + 3 * j, F);
-    if (Epart) Epart[j] = E;
+    double v[4] = {F.x, F.y, F.z, E};
+    block_sum_n<kRecvThreads, 4>(v, red);
+    if (tid == 0) {
+        st3(f.lF + 3 * j, make_double3(v[0], v[1], v[2]));
+        if (Epart) Epart[j] = v[3];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -759,9 +768,9 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
     auto M = [&](int id, int ph) { if (mark) mark(arg, id, ph); };
     M(K_SITES, 0); launch_sites(s, g, st, q); M(K_SITES, 1);
     M(K_MECH, 0); k_mech<<<nblk(s.n, 64), 64, 0, q>>>(s, g, f, E(off_mech), terms); M(K_MECH, 1);
-    if (s.n_lnk) { M(K_LINKER, 0); k_linker<<<nblk(s.n_lnk, 64), 64, 0, q>>>(s, g, st, f, E(off_lnk), terms); M(K_LINKER, 1); }
-    if (s.n_tail) { M(K_TAIL, 0); k_tail<<<nblk(s.n_tail, 64), 64, 0, q>>>(s, g, st, f, E(off_tail), terms); M(K_TAIL, 1); }
-    if (s.n_LH) { M(K_LH, 0); k_lh<<<nblk(s.n_LH, 64), 64, 0, q>>>(s, g, st, f, E(off_lh), terms); M(K_LH, 1); }
+    if (s.n_lnk) { M(K_LINKER, 0); k_linker<<<s.n_lnk, kRecvThreads, 0, q>>>(s, g, st, f, E(off_lnk), terms); M(K_LINKER, 1); }
+    if (s.n_tail) { M(K_TAIL, 0); k_tail<<<s.n_tail, kRecvThreads, 0, q>>>(s, g, st, f, E(off_tail), terms); M(K_TAIL, 1); }
+    if (s.n_LH) { M(K_LH, 0); k_lh<<<s.n_LH, kRecvThreads, 0, q>>>(s, g, st, f, E(off_lh), terms); M(K_LH, 1); }
     M(K_CORE, 0); k_core<<<s.n_c, kCoreThreads, 0, q>>>(s, g, st, f, E(off_core), terms); M(K_CORE, 1);
     if (terms & TERM_ELEC) {
         long long nn = (long long)s.n_c * s.n_c;

@@ -247,49 +247,60 @@ __device__ __forceinline__ void rpy_block(double3 ri, double3 rj, double radi, d
 }
 
 // translation_cal (func_cuda.cu:650-679) without a stored tensor: the drift
-// del * D_blk * F of bead i is summed over the beads of its species block
-// with the RPY blocks recomputed in closed form (same entries, same
-// summation order as a row of the stored D).  Beads outside the
-// hydrodynamic set (hi_all == 0: tails, LH) see their self-mobility only.
-__global__ void k_translate(Stat s, int hi_all, Geo src, Geo dst, Frc f, const double *rd,
-                            const double *rad_all) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = s.n, nt = s.n_tail, nl = s.n_LH;
-    if (b >= n + nt + nl) return;
+// del * D_blk * F of a bead is summed over the beads of its species block
+// (main / tail / LH) with the RPY blocks recomputed in closed form -- the
+// same entries a row of the stored D holds.  One workgroup per bead of
+// the hydrodynamic set, threads over partners, fixed-order reduction.
+constexpr int kTrThreads = 128;
+
+__device__ __forceinline__ void species(const Stat &s, const Geo &src, const Geo &dst, const Frc &f, int b,
+                                        const double *&x0, const double *&F, double *&out, int &lo, int &cnt) {
+    if (b < s.n) { x0 = src.r; F = f.F; out = dst.r; lo = 0; cnt = s.n; }
+    else if (b < s.n + s.n_tail) { x0 = src.r_t; F = f.tF; out = dst.r_t; lo = s.n; cnt = s.n_tail; }
+    else { x0 = src.r_lh; F = f.lF; out = dst.r_lh; lo = s.n + s.n_tail; cnt = s.n_LH; }
+}
+
+__global__ void __launch_bounds__(kTrThreads) k_translate_hi(Stat s, Geo src, Geo dst, Frc f, const double *rd,
+                                                            const double *rad_all) {
+    __shared__ double red[3 * (kTrThreads / 64)];
+    const int b = blockIdx.x;
     const double *x0, *F;
     double *out;
     int lo, cnt;
-    if (b < n) {
-        x0 = src.r; F = f.F; out = dst.r; lo = 0; cnt = n;
-    } else if (b < n + nt) {
-        x0 = src.r_t; F = f.tF; out = dst.r_t; lo = n; cnt = nt;
-    } else {
-        x0 = src.r_lh; F = f.lF; out = dst.r_lh; lo = n + nt; cnt = nl;
-    }
+    species(s, src, dst, f, b, x0, F, out, lo, cnt);
     const int i = b - lo;
-    double v[3];
-    for (int k = 0; k < 3; k++) v[k] = x0[3 * i + k] + rd[3 * b + k];
     const double radi = rad_all[b];
-    if (b < n || hi_all) {
-        const double3 ri = ld3(x0 + 3 * i);
-        for (int j = 0; j < cnt; j++) {
-            const double *Fj = F + 3 * j;
-            if (j == i) {
-                double dself = s.a1 / radi;
-                for (int k = 0; k < 3; k++)
-                    for (int l = 0; l < 3; l++) v[k] = v[k] + s.del * ((k == l) ? dself : 0.0) * Fj[l];
-                continue;
-            }
-            double M[3][3];
-            rpy_block(ri, ld3(x0 + 3 * j), radi, rad_all[lo + j], s.a1, s.a2, M);
-            for (int k = 0; k < 3; k++)
-                for (int l = 0; l < 3; l++) v[k] = v[k] + s.del * M[k][l] * Fj[l];
+    const double3 ri = ld3(x0 + 3 * i);
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+        const double *Fj = F + 3 * j;
+        if (j == i) {
+            const double dself = s.a1 / radi;
+            for (int k = 0; k < 3; k++) v[k] += s.del * dself * Fj[k];
+            continue;
         }
-    } else {
-        const double dself = s.a1 / radi;
-        for (int k = 0; k < 3; k++) v[k] = v[k] + s.del * dself * F[3 * i + k];
+        double M[3][3];
+        rpy_block(ri, ld3(x0 + 3 * j), radi, rad_all[lo + j], s.a1, s.a2, M);
+        for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 3; l++) v[k] += s.del * M[k][l] * Fj[l];
     }
-    for (int k = 0; k < 3; k++) out[3 * i + k] = v[k];
+    block_sum_n<kTrThreads, 3>(v, red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) out[3 * i + k] = (x0[3 * i + k] + rd[3 * b + k]) + v[k];
+}
+
+// beads outside the hydrodynamic set: self-mobility a1/rad only
+__global__ void k_translate_diag(Stat s, int first, Geo src, Geo dst, Frc f, const double *rd,
+                                 const double *rad_all) {
+    const int b = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= s.n + s.n_tail + s.n_LH) return;
+    const double *x0, *F;
+    double *out;
+    int lo, cnt;
+    species(s, src, dst, f, b, x0, F, out, lo, cnt);
+    const int i = b - lo;
+    const double dself = s.a1 / rad_all[b];
+    for (int k = 0; k < 3; k++) out[3 * i + k] = x0[3 * i + k] + rd[3 * b + k] + s.del * dself * F[3 * i + k];
 }
 
 // force_torque_tmp (func_cuda.cu:2626-2641)
@@ -326,8 +337,10 @@ void launch_rotate(const Stat &s, const Geo &src, const Geo &dst, const double *
 
 void launch_translate(const Stat &s, int hi_all, const Geo &src, const Geo &dst, const Frc &f,
                       const double *rd, const double *rad_all, hipStream_t q) {
-    long long tot = (long long)s.n + s.n_tail + s.n_LH;
-    k_translate<<<nblk(tot, 64), 64, 0, q>>>(s, hi_all, src, dst, f, rd, rad_all);
+    const int tot = s.n + s.n_tail + s.n_LH;
+    const int n_hi = hi_all ? tot : s.n;
+    k_translate_hi<<<n_hi, kTrThreads, 0, q>>>(s, src, dst, f, rd, rad_all);
+    if (tot > n_hi) k_translate_diag<<<nblk(tot - n_hi, 128), 128, 0, q>>>(s, n_hi, src, dst, f, rd, rad_all);
 }
 
 void launch_average(const Stat &s, const Frc &a, const Frc &b, const Frc &out, hipStream_t q) {

@@ -43,7 +43,7 @@ void launch_commit(const Stat &s, const Geo &from, const Geo &to, hipStream_t q)
 struct CholWork {
     double *A;       // mpad x ld, row-major; lower triangle holds D then L
     int m, mpad, ld;
-    double *Linv;    // 64 x 64 inverse of the current diagonal block
+    double *Linv;    // mpad: inverses of the pivots (1 / L_jj)
     int *neg;        // negative-pivot counter
     int mode;        // BD_CHOL_*
 };
