@@ -1,0 +1,150 @@
+"""Writes the golden fixtures of tests/golden/ from the compiled reference.
+
+Run in a container where the reference is present and oracle/_ref is built
+(``make -C oracle``):
+
+    python tests/golden/make_golden.py
+
+Every output here comes from the reference's own functions (func.cpp,
+mt.cpp, utilities.cpp) through oracle/_ref/libbdref.so; the inputs are
+stored beside them so the fixtures stand alone:
+  * testdata_deck.npz  -- the reference input deck test_data/ (12 cores,
+    300 charges, 600 tail beads, 12 bound linker histones) as arrays, with
+    the frame update, every force term, the 60x60 leading block of D and
+    its factor, and one full step in both hydrodynamic modes;
+  * tiny_deck.npz      -- a seeded 3-core fiber with tails and LH, same
+    content, plus a second step;
+  * overlap_tensor.npz -- beads closer than their radii (effective-radius
+    RPY branch) and a non-SPD tensor (the |f| pivot rule);
+  * rng.npz            -- genrand_int32 / genrand_real1 / rand_normal
+    sequences after init_genrand(5489).
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle_py as O  # noqa: E402
+from brownian_dynamics_cuda_amd import fiber as Fb  # noqa: E402
+from brownian_dynamics_cuda_amd import system as S  # noqa: E402
+
+REF_DECK = "/root/reference/test_data"
+
+FIBER_ARRAYS = ("types", "rad", "phi_o", "core_pos", "core_q", "tail_pos", "tail_fix_tpl", "t_q",
+                "t_rad", "h_t", "g_t", "lo_t", "beta_o_t", "t_grp", "t_fix", "LH_g_pos",
+                "LH_n_pos", "LH_c_pos", "LH_conn", "LH_q", "LH_vdw_hh", "LH_vdw_hc",
+                "LH_vdw_hl", "LH_vdw_ht", "LH_kstr", "LH_kben", "LH_streq", "LH_betaeq",
+                "LH_radius", "nc_lh_flag", "lh_rad")
+PHYS = ("T", "Cs", "Mg", "h", "g", "s", "debyell", "debye", "q_l", "k_e", "k_ex", "k_h1",
+        "sig_dd", "sig_dc", "sig_cc", "sig_tt", "sig_tl", "sig_tc", "a1", "a2", "s2dt", "dell",
+        "time_step", "kbt")
+
+
+def pack_deck(fib, st):
+    d = {f"fib_{k}": np.asarray(getattr(fib, k)) for k in FIBER_ARRAYS}
+    d.update({f"phys_{k}": np.asarray(getattr(fib.phys, k)) for k in PHYS})
+    d["fib_n_lh_n"] = np.asarray(fib.n_lh_n)
+    d["fib_n_lh_g"] = np.asarray(fib.n_lh_g)
+    d["fib_n_lh_c"] = np.asarray(fib.n_lh_c)
+    d["fib_groups_per_core"] = np.asarray(fib.groups_per_core)
+    for k in ("r", "a", "b", "c", "r_t", "r_lh"):
+        d[f"st_{k}"] = np.asarray(getattr(st, k))
+    return d
+
+
+def deck_outputs(fib, st, nsteps=1):
+    out = {}
+    ref = O.Oracle(fib, "ref")
+    fs = ref.prepare(st)
+    for k in O.STATE_FIELDS:
+        out[f"prep_{k}"] = fs.view(k).copy()
+    for term in ("mech", "elec", "tail", "lh"):
+        f = ref.forces(fs, (term,))
+        for k in ("F", "T", "tF", "lhF"):
+            out[f"force_{term}_{k}"] = f[k]
+        out[f"force_{term}_E"] = np.asarray(f["E"])
+    f = ref.forces(fs)
+    for k in ("F", "T", "tF", "lhF"):
+        out[f"force_all_{k}"] = f[k]
+    out["force_all_E"] = np.asarray(f["E"])
+    nb = min(20, fib.n)
+    D, L, _ = ref.diffusion_tensor(fs.view("r")[:3 * nb], fib.rad[:nb])
+    out["D20"], out["L20"] = D, L
+    rng = O.Rng(5489)
+    nD3 = 3 * (fib.n + fib.n_tail + fib.n_LH)
+    for hi in (0, 1):
+        fib.hi_all = hi
+        ref_h = O.Oracle(fib, "ref")
+        fs_h = fs.copy()
+        rng = O.Rng(5489)
+        for s in range(nsteps):
+            rr, p = rng.step_noise(s, fib.r_var(), nD3)
+            if hi == 0:
+                out[f"noise_rr_{s}"], out[f"noise_p_{s}"] = rr, p
+            e = ref_h.step(fs_h, rr, p)["E"]
+            out[f"step{hi}_{s}_E"] = np.asarray(e)
+            for k in O.STATE_FIELDS:
+                out[f"step{hi}_{s}_{k}"] = fs_h.view(k).copy()
+    fib.hi_all = 0
+    return out
+
+
+def overlap_case():
+    """Beads closer than the sum of their radii, and a tensor that is not
+    positive definite (pivot rule |f|)."""
+    fib, _ = Fb.make_fiber(3, seed=1)
+    ref = O.Oracle(fib, "ref")
+    r = np.array([0.0, 0.0, 0.0, 1.0, 0.5, 0.0, 2.2, 0.4, 0.3, 9.0, 0.0, 0.0])
+    rad = np.array([1.5, 1.5, 5.0, 1.5])
+    D, L, _ = ref.diffusion_tensor(r, rad)
+    # many strongly overlapping large beads: the RPY overlap branch is then
+    # not positive definite
+    rng = np.random.default_rng(3)
+    r2 = rng.uniform(0.0, 1.2, size=(8, 3)).reshape(-1)
+    rad2 = np.full(8, 5.0)
+    D2, L2, _ = ref.diffusion_tensor(r2, rad2)
+    return dict(r=r, rad=rad, D=D, L=L, r2=r2, rad2=rad2, D2=D2, L2=L2,
+                a1=np.asarray(fib.phys.a1), a2=np.asarray(fib.phys.a2))
+
+
+def rng_case():
+    lib = O.ref_lib()
+    lib.ref_init_genrand(5489)
+    u = np.array([lib.ref_genrand_real1() for _ in range(64)])
+    lib.ref_init_genrand(5489)
+    sds = np.array([1.0, 2.0, 0.5, 3.0] * 16)
+    z = np.array([lib.ref_rand_normal(0.25, sd) for sd in sds])
+    # the polar cache is process-global in the reference: leave it empty
+    if len(sds) % 2:
+        lib.ref_rand_normal(0, 1)
+    return dict(real1=u, normal_sd=sds, normal=z)
+
+
+def main():
+    if not O.ref_available():
+        sys.exit("oracle/_ref/libbdref.so is missing: build it with `make -C oracle` "
+                 "where the reference is present")
+    if os.path.isdir(REF_DECK):
+        fib, st = S.load_reference_deck(REF_DECK)
+        d = pack_deck(fib, st)
+        d.update(deck_outputs(fib, st, nsteps=1))
+        np.savez_compressed(os.path.join(HERE, "testdata_deck.npz"), **d)
+    fib, st = Fb.make_fiber(3, seed=11, with_lh=True)
+    fib.groups_per_core = 10  # reference literal (func.cpp:2368); 5 groups -> core k//2
+    d = pack_deck(fib, st)
+    d.update(deck_outputs(fib, st, nsteps=2))
+    np.savez_compressed(os.path.join(HERE, "tiny_deck.npz"), **d)
+    np.savez_compressed(os.path.join(HERE, "overlap_tensor.npz"), **overlap_case())
+    np.savez_compressed(os.path.join(HERE, "rng.npz"), **rng_case())
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()
