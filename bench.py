@@ -40,7 +40,30 @@ def parse():
     ap.add_argument("--timing-steps", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=2, help="reference CPU steps to time (0: skip)")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the step's kernels one by one instead of replaying the "
+                         "captured hipGraph (same kernels; for profilers)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the timed steps, run the library's streaming copy kernel "
+                         "once over MIB MiB (calibrates HBM traffic counters)")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles",
+                                                           "hbm_traffic_cfg2.txt"),
+                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the roofline traffic")
     return ap.parse_args()
+
+
+def read_traffic(path, kernel):
+    """HBM bytes per launch of `kernel` from a committed counter summary
+    (lines 'traffic <kernel> <bytes_per_launch>'); None if absent."""
+    try:
+        with open(path) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 3 and w[0] == "traffic" and w[1] == kernel:
+                    return float(w[2])
+    except OSError:
+        pass
+    return None
 
 
 def noise_stream(fib, nsteps, seed=5489):
@@ -85,7 +108,7 @@ def main():
     t_noise = time.time() - t0
     n3, nD3 = rr.shape[1], p.shape[1]
 
-    eng = E.BDEngine(fib, chol_mode=E.CHOL_FP64, use_graph=True)
+    eng = E.BDEngine(fib, chol_mode=E.CHOL_FP64, use_graph=not args.eager)
     eng.set_state(st, rebuild=True)
     d_rr = E.DeviceBuffer(rr.nbytes)
     d_p = E.DeviceBuffer(p.nbytes)
@@ -108,6 +131,12 @@ def main():
     finite = bool(np.all(np.isfinite(final.r)) and np.all(np.isfinite(energies)))
     ms = 1e3 * dt / K
     steps_per_s = K / dt
+    if args.copy_probe > 0:
+        nb = args.copy_probe << 20
+        src, dst = E.DeviceBuffer(nb), E.DeviceBuffer(nb)
+        dst.copy_from(src, nb)
+        src.free()
+        dst.free()
 
     # per-kernel HIP-event timing on the step's stream (a few instrumented steps)
     eng.set_timing(True)
@@ -126,16 +155,20 @@ def main():
         avg_s = tot_ms / launches / 1e3
         per_launch = flops / (launches / args.timing_steps)
         roof = dict(bound="mfma", achieved=per_launch / avg_s / 1e12, peak=PEAK_FP64 / 1e12,
-                    unit="TFLOP/s", traffic=None, kernel=dominant)
+                    unit="TFLOP/s", kernel=dominant, pipe="FP64 MFMA",
+                    flop_per_launch=per_launch, avg_launch_us=avg_s * 1e6)
     else:
-        # core-core charge sums: FLOPs counted per charge pair
+        # Debye-Hueckel charge sums: 38 FLOPs per charge pair, FP64 on the
+        # vector ALUs, whose MI355X peak equals the FP64 matrix peak
         pair_flops = 38.0
         tot_ms, launches = kt[dominant]
         avg_s = tot_ms / launches / 1e3
         per_launch = counts["core_charge_pairs"] * pair_flops
-        roof = dict(bound="fp64", achieved=per_launch / avg_s / 1e12, peak=PEAK_FP64 / 1e12,
-                    unit="TFLOP/s", traffic=None, kernel=dominant)
+        roof = dict(bound="mfma", achieved=per_launch / avg_s / 1e12, peak=PEAK_FP64 / 1e12,
+                    unit="TFLOP/s", kernel=dominant, pipe="FP64 VALU",
+                    flop_per_launch=per_launch, avg_launch_us=avg_s * 1e6)
     roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = read_traffic(args.traffic_file, dominant)
 
     # mixed-precision factor: same state and noise, one step each way
     mixed = None

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -105,6 +106,12 @@ struct bd_context {
 };
 
 namespace {
+
+__global__ void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, long long nw) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nw;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
 
 void mark_cb(void *arg, int id, int phase) {
     bd_context *c = static_cast<bd_context *>(arg);
@@ -750,6 +757,19 @@ int bd_memcpy_htod(void *dst, const void *src, size_t bytes) {
 }
 int bd_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
     HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return BD_OK;
+}
+int bd_device_copy(void *dst, const void *src, size_t bytes) {
+    if (!dst || !src) return fail(BD_E_ARG, "null argument");
+    if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+        return fail(BD_E_ARG, "bd_device_copy needs 16-byte aligned pointers and sizes");
+    const long long nw = (long long)(bytes / 16);
+    if (nw == 0) return BD_OK;
+    const long long blocks = std::min<long long>((nw + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, 0, (uint4 *)dst,
+                       (const uint4 *)src, nw);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
     return BD_OK;
 }
 int bd_synchronize(bd_context *c) {

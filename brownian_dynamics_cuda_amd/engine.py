@@ -90,6 +90,7 @@ API = {
     "bd_device_free": ([C.c_void_p], C.c_int),
     "bd_memcpy_htod": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
     "bd_memcpy_dtoh": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
+    "bd_device_copy": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
     "bd_synchronize": ([C.c_void_p], C.c_int),
     "bd_rng_create": ([C.c_uint, C.POINTER(C.c_void_p)], C.c_int),
     "bd_rng_destroy": ([C.c_void_p], C.c_int),
@@ -356,6 +357,12 @@ class DeviceBuffer:
         _check(self.lib.bd_memcpy_dtoh(out.ctypes.data, C.c_void_p(self.ptr + offset),
                                        out.nbytes), "bd_memcpy_dtoh")
         return out
+
+    def copy_from(self, other: "DeviceBuffer", nbytes: int):
+        """Device-to-device copy of nbytes (the library's streaming copy kernel)."""
+        assert nbytes <= min(self.nbytes, other.nbytes)
+        _check(self.lib.bd_device_copy(C.c_void_p(self.ptr), C.c_void_p(other.ptr), nbytes),
+               "bd_device_copy")
 
     def free(self):
         if self.ptr:

@@ -171,6 +171,10 @@ int bd_device_alloc(size_t bytes, void **ptr);
 int bd_device_free(void *ptr);
 int bd_memcpy_htod(void *dst, const void *src, size_t bytes);
 int bd_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte words; both
+ * pointers 16-byte aligned, bytes a multiple of 16); synchronous.  Also the
+ * known-size copy that HBM traffic counters are calibrated against. */
+int bd_device_copy(void *dst, const void *src, size_t bytes);
 int bd_synchronize(bd_context *ctx);
 
 /* Host noise: MT19937 + Marsaglia polar normals with the cached second
