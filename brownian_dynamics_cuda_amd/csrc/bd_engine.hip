@@ -6,7 +6,9 @@
 //   * every buffer is allocated once at bd_create (the reference frees and
 //     re-allocates LH arrays, D, its factor and the noise buffers each step,
 //     func_cuda.cu:3436-3548) -- a step allocates nothing;
-//   * one HIP stream; the whole step can be captured once and replayed as a
+//   * two HIP streams: D, its Cholesky factor and rd are formed on a side
+//     stream while force pass 1 runs on the main one (both only read the
+//     state at x0); the whole step can be captured once and replayed as a
 //     hipGraph (no hipMemsetAsync anywhere in it, counters are cleared by
 //     fill kernels);
 //   * D is assembled and factored in place (lower triangle); the drift
@@ -60,6 +62,11 @@ const char *kKernelNames[K_COUNT] = {
 struct bd_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream of the step: D, its factor and rd run there, beside
+    // force pass 1 (both read only the state at x0); joined before the
+    // predictor translation
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<void *> allocs;
     // sizes
     int n = 0, n_c = 0, n_tail = 0, n_LH = 0, lh_cap = 0, n_D = 0, n_D3 = 0, n_hi = 0, hi_all = 0;
@@ -153,12 +160,12 @@ void enqueue_forces(bd_context *c, const Geo &g, const Frc &f, double *Epart, in
     launch_forces(c->st, g, c->sites, f, c->fw, Epart, terms, c->stream, c->timing ? mark_cb : nullptr, c);
 }
 
-void enqueue_factor(bd_context *c, const Geo &g) {
+void enqueue_factor(bd_context *c, const Geo &g, hipStream_t q) {
     Marker M{c};
     M(K_RPY, 0);
-    launch_rpy(c->st, c->n_hi, g, c->rad_all, c->chol, c->stream);
+    launch_rpy(c->st, c->n_hi, g, c->rad_all, c->chol, q);
     M(K_RPY, 1);
-    launch_cholesky(c->chol, c->stream, M.fn(), c);
+    launch_cholesky(c->chol, q, M.fn(), c);
 }
 
 // One predictor-corrector step (func_cuda.cu:3494-3660) reading c->rr, c->p.
@@ -166,16 +173,27 @@ void enqueue_step(bd_context *c) {
     Marker M{c};
     hipStream_t q = c->stream;
     const Stat &s = c->st;
+    // D, its factor and the noise displacement (3504-3552) on the side
+    // stream (serial when the per-kernel timing markers are on)
+    hipStream_t qf = q;
+    if (!c->timing) {
+        (void)hipEventRecord(c->ev_fork, q);
+        (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
+        qf = c->side;
+    }
+    enqueue_factor(c, c->cur, qf);
+    M(K_NOISE, 0); launch_noise(s, c->n_hi, c->n_D3, c->chol, c->p, c->rad_all, c->rd, qf); M(K_NOISE, 1);
     // LH bending angles at the current state (3497)
     M(K_BETAS, 0); launch_betas(s, c->cur, 0, 1, q); M(K_BETAS, 1);
-    // D, its factor and the noise displacement (3504-3552)
-    enqueue_factor(c, c->cur);
-    M(K_NOISE, 0); launch_noise(s, c->n_hi, c->n_D3, c->chol, c->p, c->rad_all, c->rd, q); M(K_NOISE, 1);
     // force pass 1 at x0 and its energy (3556-3615)
     enqueue_forces(c, c->cur, c->f0, c->Epart, BD_TERM_ALL);
     M(K_EREDUCE, 0); launch_energy_reduce(c->Epart, c->ne, c->energy, q); M(K_EREDUCE, 1);
     // predictor (3617-3619)
     M(K_ROTATE, 0); launch_rotate(s, c->cur, c->pred, c->f0.T, c->rr, q); M(K_ROTATE, 1);
+    if (qf != q) {
+        (void)hipEventRecord(c->ev_join, qf);
+        (void)hipStreamWaitEvent(q, c->ev_join, 0);
+    }
     M(K_TRANSLATE, 0); launch_translate(s, c->hi_all, c->cur, c->pred, c->f0, c->rd, c->rad_all, q); M(K_TRANSLATE, 1);
     // rebuild at the predicted state (3621-3625)
     M(K_BETAS, 0); launch_betas(s, c->pred, 1, 1, q); M(K_BETAS, 1);
@@ -286,8 +304,12 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     const int nD_cap = n + nt + c->lh_cap;   // buffers sized for the LH capacity
     const int nhi_cap = c->hi_all ? nD_cap : n;
     auto bail = [&](int r) { bd_destroy(c); return r; };
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(BD_E_HIP, "hipStreamCreate failed"));
+    if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(BD_E_HIP, "hipEventCreate failed"));
 
     // host-side index tables
     std::vector<int> ic_of(n), core_bead, elig(n), lnk, lhc_bead, lh_of_core(n_c, -1);
@@ -453,7 +475,11 @@ int bd_destroy(bd_context *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (auto &pe : c->pending) (void)hipEventDestroy(pe.second);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     for (void *p : c->allocs) (void)hipFree(p);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return BD_OK;
@@ -650,7 +676,7 @@ int bd_factor(bd_context *c) {
     if (!c->state_set) return fail(BD_E_STATE, "bd_set_state must be called first");
     HIPCHK(hipSetDevice(c->device));
     launch_betas(c->st, c->cur, 0, 1, c->stream);
-    enqueue_factor(c, c->cur);
+    enqueue_factor(c, c->cur, c->stream);
     HIPCHK(hipGetLastError());
     if (c->timing) collect_timing(c);
     return sync_check(c);
