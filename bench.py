@@ -149,7 +149,9 @@ def main():
     dominant = max(per_step, key=per_step.get)
 
     # roofline of the dominant kernel
-    if dominant in ("syrk", "trsm", "potrf"):
+    if dominant in ("chol", "syrk", "trsm", "potrf"):
+        # the factorisation's algorithmic FLOPs (n^3/3 per factor; the
+        # trailing-update share when the per-panel-step kernels are used)
         flops = syrk_flops_total(n_hi3) if dominant == "syrk" else chol_flops(n_hi3)
         tot_ms, launches = kt[dominant]
         avg_s = tot_ms / launches / 1e3

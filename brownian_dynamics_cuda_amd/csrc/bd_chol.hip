@@ -7,11 +7,13 @@
 //
 // Matrix: row-major, lower triangle, padded to a multiple of 64 with an
 // identity block (so the padded factor is [[L,0],[0,I]] and no kernel needs
-// a bounds test).  Right-looking blocked algorithm with 64x64 tiles:
-//   panel kb:  trsm   L_ib,kb = A_ib,kb * inv(L_kb,kb)^T   (FP64 MFMA)
-//              syrk   A_ib,jb -= L_ib,kb * L_jb,kb^T       (FP64 or FP32 MFMA)
-//              and the workgroup owning tile (kb+1,kb+1) factors it in LDS
-//              right after its update (potrf + triangular inverse, FP64).
+// a bounds test).  Right-looking blocked algorithm with 64x64 tiles, two
+// launches per block column kb:
+//   trsm   L_ib,kb L_kb,kb^T = A_ib,kb           (substitution, a wavefront
+//          per row tile, FP64)
+//   syrk   A_ib,jb -= L_ib,kb * L_jb,kb^T        (FP64 or FP32 matrix cores)
+//          and the workgroup owning tile (kb+1,kb+1) factors it right after
+//          its update (one wavefront, FP64).
 #include "bd_dev.h"
 #include "bd_kernels.h"
 
@@ -164,114 +166,178 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
-// Cholesky of the 64x64 LDS tile L (lower, in place), the CPU algorithm of
-// func.cpp:806-836 reordered into 16-column sub-panels.  A pivot f < 0 is
-// replaced by -f and counted (func.cpp:827-833); the pivot's inverse is
-// kept in invd[] for the triangular solves.  Each sub-panel is factored by
-// wavefront 0 alone -- lane r holds row r, the 16 columns in registers,
-// pivots and column entries are broadcast with readlane, so a column step
-// has no barrier -- then all waves apply its rank-16 update to the rest of
-// the tile.  Requires blockDim.x == 256.
-__device__ void potrf_tile_lds(double *L, double *invd, int *neg) {
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (int sp = 0; sp < 4; sp++) {
-        const int c0 = 16 * sp;
-        if (w == 0) {
-            double l[16];
-            const bool act = lane >= c0;
+// ---------------------------------------------------------------------------
+// Diagonal tile factor (one wavefront) and the panel solve, one row per
+// lane in registers, in panels of 8 columns.
+//
+// LP holds the factor of the diagonal tile in panel order: LP[p][c][q] =
+// L[c][8p + q] (8 panels x 64 rows x 8 doubles, 32 KB), so the rank-8
+// update of a row reads the 8 panel entries of column c as 4 broadcast
+// 16-byte LDS loads and applies 8 FMAs.
+// ---------------------------------------------------------------------------
+// Panel P0..P0+7 of the factor of a 64x64 tile held one row per lane in
+// a[] (entries right of the diagonal are ignored): the CPU algorithm of
+// func.cpp:806-836 in right-looking order -- pivot f (|f| and a count if
+// f < 0), column scaled by 1/sqrt(f) -- with the updates inside the panel
+// taken from the owning lanes by readlane, then the panel published to LP
+// and the rank-8 update of the columns to its right.  Lane j keeps 1/L_jj
+// in myinv.  Branch-free, so the row stays in registers.
+template <int P0>
+__device__ __forceinline__ void factor_panel8(double (&a)[TB], double *LP, double &myinv, int &nneg) {
+    const int lane = threadIdx.x & 63;
+    double *lp = LP + (P0 / 8) * (TB * 8);
 #pragma unroll
-            for (int k = 0; k < 16; k++) l[k] = act ? L[lane * LDP + c0 + k] : 0.0;
+    for (int j = P0; j < P0 + 8; j++) {
+        const double f0 = readlane_f64(a[j], j);
+        nneg += f0 < 0 ? 1 : 0;
+        const double f = fabs(f0);
+        const double inv = rsqrt_f64(f);
+        const double lj = lane > j ? a[j] * inv : (lane == j ? f * inv : 0.0);
+        myinv = lane == j ? inv : myinv;
+        a[j] = lj;
 #pragma unroll
-            for (int jj = 0; jj < 16; jj++) {
-                const int j = c0 + jj;
-                double f = readlane_f64(l[jj], j);
-                if (f < 0) {
-                    f = -f;
-                    if (lane == 0) atomicAdd(neg, 1);
-                }
-                const double inv = rsqrt_f64(f);
-                if (lane == j) { l[jj] = f * inv; invd[j] = inv; }
-                else if (lane > j) l[jj] = l[jj] * inv;
-#pragma unroll
-                for (int kk = jj + 1; kk < 16; kk++) {
-                    const double ckj = readlane_f64(l[jj], c0 + kk);  // L[c0+kk][j]
-                    if (lane >= c0 + kk) l[kk] -= l[jj] * ckj;
-                }
-            }
-            if (act)
-#pragma unroll
-                for (int k = 0; k < 16; k++) L[lane * LDP + c0 + k] = l[k];
-        }
-        __syncthreads();
-        // rank-16 update of the tile below and right of the sub-panel
-        const int base = c0 + 16;
-        const int nr = TB - base;
-        for (int e = tid; e < nr * nr; e += blockDim.x) {
-            const int r = base + e / nr, k = base + e % nr;
-            if (k > r) continue;
-            double acc = L[r * LDP + k];
-#pragma unroll
-            for (int c = 0; c < 16; c++) acc -= L[r * LDP + c0 + c] * L[k * LDP + c0 + c];
-            L[r * LDP + k] = acc;
-        }
-        __syncthreads();
+        for (int c = j + 1; c < P0 + 8; c++) a[c] -= lj * readlane_f64(lj, c);
     }
+#pragma unroll
+    for (int q = 0; q < 8; q += 2) {
+        double2 v;
+        v.x = a[P0 + q];
+        v.y = a[P0 + q + 1];
+        *reinterpret_cast<double2 *>(lp + lane * 8 + q) = v;
+    }
+#pragma unroll
+    for (int c = P0 + 8; c < TB; c++) {
+        double acc = a[c];
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc -= a[P0 + q] * lp[c * 8 + q];
+        a[c] = acc;
+    }
+}
+
+// Panel P0..P0+7 of the row solve x L^T = b (row per lane in b[]) with the
+// factor in LP and the pivot inverses in iv.
+template <int P0>
+__device__ __forceinline__ void solve_panel8(double (&b)[TB], const double *LP, const double *iv) {
+    const double *lp = LP + (P0 / 8) * (TB * 8);
+#pragma unroll
+    for (int j = P0; j < P0 + 8; j++) {
+        const double xj = b[j] * iv[j];
+        b[j] = xj;
+#pragma unroll
+        for (int c = j + 1; c < P0 + 8; c++) b[c] -= xj * lp[c * 8 + (j - P0)];
+    }
+#pragma unroll
+    for (int c = P0 + 8; c < TB; c++) {
+        double acc = b[c];
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc -= b[P0 + q] * lp[c * 8 + q];
+        b[c] = acc;
+    }
+}
+
+__device__ __forceinline__ void load_row(const double *A, int ld, int r0, int c0, double (&b)[TB]) {
+    const double *row = A + (size_t)(r0 + (threadIdx.x & 63)) * ld + c0;
+#pragma unroll
+    for (int c = 0; c < TB; c += 2) {
+        const double2 v = *reinterpret_cast<const double2 *>(row + c);
+        b[c] = v.x;
+        b[c + 1] = v.y;
+    }
+}
+
+__device__ __forceinline__ void store_row(double *A, int ld, int r0, int c0, const double (&b)[TB], bool lower) {
+    const int lane = threadIdx.x & 63;
+    double *row = A + (size_t)(r0 + lane) * ld + c0;
+#pragma unroll
+    for (int c = 0; c < TB; c += 2) {
+        double2 v;
+        v.x = (!lower || c <= lane) ? b[c] : 0.0;
+        v.y = (!lower || c + 1 <= lane) ? b[c + 1] : 0.0;
+        *reinterpret_cast<double2 *>(row + c) = v;
+    }
+}
+
+// Wavefront 0 of the calling workgroup factors the tile T (LDS, pitch LDP)
+// at (d0, d0): L goes to A (lower, zeros above), the factor in panel order
+// to Lp (global copy of LP for the panel solves) and the pivot inverses to
+// invd.  LP (64*64) is LDS scratch.
+__device__ __forceinline__ void diag_factor(const double *T, double *LP, double *A, int ld, int d0, double *Lp,
+                                            double *invd, int *neg) {
+    const int lane = threadIdx.x & 63;
+    double a[TB];
+#pragma unroll
+    for (int c = 0; c < TB; c++) a[c] = T[lane * LDP + c];
+    int nneg = 0;
+    double myinv = 0.0;
+    factor_panel8<0>(a, LP, myinv, nneg);
+    factor_panel8<8>(a, LP, myinv, nneg);
+    factor_panel8<16>(a, LP, myinv, nneg);
+    factor_panel8<24>(a, LP, myinv, nneg);
+    factor_panel8<32>(a, LP, myinv, nneg);
+    factor_panel8<40>(a, LP, myinv, nneg);
+    factor_panel8<48>(a, LP, myinv, nneg);
+    factor_panel8<56>(a, LP, myinv, nneg);
+    store_row(A, ld, d0, d0, a, true);
+#pragma unroll 8
+    for (int e = 0; e < TB * TB / 2; e += TB)
+        reinterpret_cast<double2 *>(Lp)[e + lane] = reinterpret_cast<const double2 *>(LP)[e + lane];
+    invd[d0 + lane] = myinv;
+    if (nneg && lane == 0) atomicAdd(neg, nneg);
 }
 
 // first diagonal tile
-__global__ void __launch_bounds__(256) k_potrf0(double *A, int ld, double *invd, int *neg) {
-    __shared__ double L[TB * LDP];
-    load_tile(A, ld, 0, 0, L);
+__global__ void __launch_bounds__(256) k_potrf0(double *A, int ld, double *Lp, double *invd, int *neg) {
+    __shared__ double T[TB * LDP], LP[TB * TB];
+    load_tile(A, ld, 0, 0, T);
     __syncthreads();
-    potrf_tile_lds(L, invd, neg);
-    store_tile(A, ld, 0, 0, L, true);
+    if ((threadIdx.x >> 6) == 0) diag_factor(T, LP, A, ld, 0, Lp, invd, neg);
 }
 
-// Panel below the diagonal: X * L_kk^T = A_ib,kb for every row block
-// ib > kb, by forward substitution.  Lane = row (64 rows per wavefront,
-// four wavefronts per workgroup); the row lives in registers; L_kk and
-// the pivot inverses are broadcast from LDS.
-__global__ void __launch_bounds__(256) k_trsm(double *A, int ld, int kb, const double *invd, int nrow_blocks) {
-    __shared__ double Lk[TB * TB], iv[TB];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (int e = tid; e < TB * TB; e += blockDim.x) {
-        const int r = e / TB, c = e % TB;
-        Lk[e] = (c <= r) ? A[(size_t)(kb * TB + r) * ld + kb * TB + c] : 0.0;
-    }
+// Panel below the diagonal: X L_kb,kb^T = A_ib,kb for the row tiles
+// ib > kb, one tile per wavefront (four per workgroup), one row per lane,
+// by substitution against the diagonal factor in panel order.
+__global__ void __launch_bounds__(256) k_trsm(double *A, int ld, int kb, const double *Lp, const double *invd,
+                                              int nrow_blocks) {
+    __shared__ double LP[TB * TB], iv[TB];
+    const int tid = threadIdx.x, w = tid >> 6;
+    for (int e = tid; e < TB * TB / 2; e += blockDim.x)
+        reinterpret_cast<double2 *>(LP)[e] = reinterpret_cast<const double2 *>(Lp + (size_t)kb * TB * TB)[e];
     if (tid < TB) iv[tid] = invd[kb * TB + tid];
     __syncthreads();
-    const int ib = kb + 1 + blockIdx.x * 4 + w;
-    if (blockIdx.x * 4 + w >= nrow_blocks) return;
-    double *row = A + (size_t)(ib * TB + lane) * ld + kb * TB;
+    const int t = blockIdx.x * 4 + w;
+    if (t >= nrow_blocks) return;
+    const int ib = kb + 1 + t;
     double b[TB];
-#pragma unroll
-    for (int k = 0; k < TB; k++) b[k] = row[k];
-#pragma unroll
-    for (int j = 0; j < TB; j++) {
-        b[j] = b[j] * iv[j];
-#pragma unroll
-        for (int k = j + 1; k < TB; k++) b[k] -= Lk[k * TB + j] * b[j];
-    }
-#pragma unroll
-    for (int k = 0; k < TB; k++) row[k] = b[k];
+    load_row(A, ld, ib * TB, kb * TB, b);
+    solve_panel8<0>(b, LP, iv);
+    solve_panel8<8>(b, LP, iv);
+    solve_panel8<16>(b, LP, iv);
+    solve_panel8<24>(b, LP, iv);
+    solve_panel8<32>(b, LP, iv);
+    solve_panel8<40>(b, LP, iv);
+    solve_panel8<48>(b, LP, iv);
+    solve_panel8<56>(b, LP, iv);
+    store_row(A, ld, ib * TB, kb * TB, b, false);
 }
 
-// trailing update of tile (ib, jb), kb < jb <= ib; the tile (kb+1, kb+1) is
-// then factored in place.  FP32 MFMA variant when MODE != 0.
+// Trailing update of tile (ib, jb), kb < jb <= ib; the workgroup of tile
+// (kb+1, kb+1) then factors it (wavefront 0).  FP32 matrix
+// cores with FP32 accumulation when MODE != 0.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_syrk(double *A, int ld, int kb, double *invd, int *neg) {
+__global__ void __launch_bounds__(256) k_syrk(double *A, int ld, int kb, double *Lp, double *invd, int *neg) {
     __shared__ double As[TB * LDP], Bs[TB * LDP];
     int ii, jj;
     tri_index(blockIdx.x, ii, jj);
     const int ib = kb + 1 + ii, jb = kb + 1 + jj;
     load_tile(A, ld, ib * TB, kb * TB, As);
-    load_tile(A, ld, jb * TB, kb * TB, Bs);
+    if (ib != jb) load_tile(A, ld, jb * TB, kb * TB, Bs);
     __syncthreads();
+    const double *Bp = (ib != jb) ? Bs : As;
     if (MODE == 0) {
         dbl4 acc[2][2];
         for (int a = 0; a < 2; a++)
             for (int b = 0; b < 2; b++) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-        mma_f64(As, Bs, acc);
+        mma_f64(As, Bp, acc);
         __syncthreads();
         load_tile(A, ld, ib * TB, jb * TB, As);
         __syncthreads();
@@ -280,7 +346,7 @@ __global__ void __launch_bounds__(256) k_syrk(double *A, int ld, int kb, double 
         flt4 acc[2][2];
         for (int a = 0; a < 2; a++)
             for (int b = 0; b < 2; b++) acc[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
-        mma_f32(As, Bs, acc);
+        mma_f32(As, Bp, acc);
         __syncthreads();
         load_tile(A, ld, ib * TB, jb * TB, As);
         __syncthreads();
@@ -288,8 +354,7 @@ __global__ void __launch_bounds__(256) k_syrk(double *A, int ld, int kb, double 
     }
     __syncthreads();
     if (ii == 0 && jj == 0) {
-        potrf_tile_lds(As, invd + ib * TB, neg);
-        store_tile(A, ld, ib * TB, jb * TB, As, true);
+        if ((threadIdx.x >> 6) == 0) diag_factor(As, Bs, A, ld, ib * TB, Lp + (size_t)ib * TB * TB, invd, neg);
     } else {
         store_tile(A, ld, ib * TB, jb * TB, As, ib == jb);
     }
@@ -326,18 +391,18 @@ void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_a
     auto M_ = [&](int id, int ph) { if (mark) mark(mark_arg, id, ph); };
     launch_fill_int(c.neg, 1, 0, q);
     M_(K_POTRF, 0);
-    k_potrf0<<<1, 256, 0, q>>>(c.A, c.ld, c.Linv, c.neg);
+    k_potrf0<<<1, 256, 0, q>>>(c.A, c.ld, c.Lp, c.invd, c.neg);
     M_(K_POTRF, 1);
     for (int kb = 0; kb + 1 < nt; kb++) {
         const int M = nt - kb - 1;
         M_(K_TRSM, 0);
-        k_trsm<<<(M + 3) / 4, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, M);
+        k_trsm<<<(M + 3) / 4, 256, 0, q>>>(c.A, c.ld, kb, c.Lp, c.invd, M);
         M_(K_TRSM, 1);
         M_(K_SYRK, 0);
         if (c.mode == 0)
-            k_syrk<0><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
+            k_syrk<0><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Lp, c.invd, c.neg);
         else
-            k_syrk<1><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
+            k_syrk<1><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Lp, c.invd, c.neg);
         M_(K_SYRK, 1);
     }
 }
