@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--config", type=int, default=2, help="benchmark configuration (1-5)")
     ap.add_argument("--timing-steps", type=int, default=5)
+    ap.add_argument("--host-io-steps", type=int, default=50,
+                    help="steps timed through bd_step with host noise buffers (PCIe-inclusive rate)")
     ap.add_argument("--cpu-steps", type=int, default=2, help="reference CPU steps to time (0: skip)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--eager", action="store_true",
@@ -137,6 +139,16 @@ def main():
         dst.copy_from(src, nb)
         src.free()
         dst.free()
+
+    # PCIe-inclusive rate: bd_step copies rr and p from host memory and the
+    # energy back every step (the func_cuda.h path also copies r_lh back)
+    host_io = None
+    if args.host_io_steps > 0:
+        nh = min(args.host_io_steps, W + K)
+        t1 = time.perf_counter()
+        for s in range(nh):
+            eng.step(rr[s], p[s])
+        host_io = nh / (time.perf_counter() - t1)
 
     # per-kernel HIP-event timing on the step's stream (a few instrumented steps)
     eng.set_timing(True)
@@ -239,6 +251,7 @@ def main():
         "kernels_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items(), key=lambda x: -x[1])},
         "work": counts,
         "finite": finite,
+        "pcie_inclusive_steps_per_s": host_io,
         "noise_gen_s": round(t_noise, 2),
     }
     if cpu:
