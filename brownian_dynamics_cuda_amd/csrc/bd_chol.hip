@@ -152,14 +152,6 @@ __device__ __forceinline__ void acc_to_lds(double *C, V acc[2][2], double sign, 
             }
 }
 
-// 1/sqrt(f) to full FP64 accuracy: v_rsq_f64 refined by two Newton steps.
-__device__ __forceinline__ double rsqrt_f64(double f) {
-    double y = __builtin_amdgcn_rsq(f);
-    y = y * (1.5 - 0.5 * f * y * y);
-    y = y * (1.5 - 0.5 * f * y * y);
-    return y;
-}
-
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
     int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
     int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);

@@ -88,6 +88,13 @@ __device__ __forceinline__ double dot(double3 u, double3 v) { return u.x * v.x +
 __device__ __forceinline__ double3 cross(double3 u, double3 v) {
     return make_double3(u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x);
 }
+// 1/sqrt(f) to full FP64 accuracy: v_rsq_f64 refined by two Newton steps.
+__device__ __forceinline__ double rsqrt_f64(double f) {
+    double y = __builtin_amdgcn_rsq(f);
+    y = y * (1.5 - 0.5 * f * y * y);
+    y = y * (1.5 - 0.5 * f * y * y);
+    return y;
+}
 __device__ __forceinline__ double3 zero3() { return make_double3(0.0, 0.0, 0.0); }
 
 // r + a*p.x + b*p.y + c*p.z (body point p of a frame at r)

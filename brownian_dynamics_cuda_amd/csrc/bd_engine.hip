@@ -53,7 +53,7 @@ int fail(int code, const char *fmt, ...) {
     } while (0)
 
 const char *kKernelNames[K_COUNT] = {
-    "sites", "mech", "linker", "tail", "lh", "core", "cc_list", "cc_work", "cc_reduce", "rpy",
+    "sites", "mech", "linker", "tail", "lh", "core", "cc_list", "cc_work", "rpy",
     "potrf", "trsm", "syrk", "noise", "frames", "betas", "rotate", "translate", "average",
     "commit", "energy_reduce"};
 
@@ -275,7 +275,7 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
         ncount += sys->type[i];
     }
     if (ncount != n_c) return fail(BD_E_ARG, "n_c=%d but type[] has %d cores", n_c, ncount);
-    if (Nq < 0 || Nq > 4096) return fail(BD_E_ARG, "Nq=%d out of range", Nq);
+    if (Nq < 0 || Nq > 640) return fail(BD_E_ARG, "Nq=%d out of range (at most 640 charges per core)", Nq);
     if (sys->n_lh_g > kMaxGSites) return fail(BD_E_ARG, "n_lh_g=%d exceeds %d", sys->n_lh_g, kMaxGSites);
     const int nlh = sys->n_lh_n + sys->n_lh_c;
     if (sys->n_LH > 0 && sys->n_lh_n != 0)

@@ -6,7 +6,7 @@
 // FP64 sum with no atomics.  Pairs that contribute to two receivers are
 // evaluated from both sides (cheap short-range terms) except the
 // dominant core-core charge sum, which is evaluated once per core pair
-// into a slot matrix and gathered by bd_cc_reduce.
+// into a slot matrix and gathered in index order by the core kernel.
 //
 // Energies: every pair/bond energy is counted once, by a fixed owner
 // (see the comment at each term), into per-thread slots of Epart.
@@ -437,76 +437,153 @@ __global__ void __launch_bounds__(kRecvThreads) k_lh(Stat s, Geo g, Sites st, Fr
 }
 
 // ---------------------------------------------------------------------------
-// Workgroup per core (ordinal kc): everything acting on the core body except
-// the core-core charge sum.  F[cb] +=, T[cb] +=.
-// Energy owner: core-linker charge pairs, LHG-* pairs of this core's head
-// (LHG-LHG only towards higher LH ordinals), the LHG-C spring.
+// Workgroup per core (ordinal kc): everything acting on the core body.
+// F[cb] +=, T[cb] +=.  It runs after k_cc_work and also gathers this core's
+// row (as c1) and column (as c2) of the core-core slot matrix.
+// Energy owners: core-linker charge pairs, LHG-* pairs of this core's head
+// (LHG-LHG only towards higher LH ordinals), the LHG-C spring (slot
+// Ecore[kc]); core-core pairs with this core as the lower index (Ecc[kc]).
+//
+// Partner lists are culled by the whole workgroup, 320 candidates at a
+// time, into LDS in index order (ballot + prefix count), so the per-charge
+// loops visit the surviving partners in the reference's order from LDS.
 // ---------------------------------------------------------------------------
 constexpr int kCoreThreads = 320;
-constexpr int kMaxG = 8;
+constexpr int kCoreQPer = 2;   // charges per thread: Nq <= 640
+constexpr int kMaxRT = 8;      // redirected tails summed per pass over the LH partners
 
-__global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
+// Ordered compaction of one candidate per thread: returns the number kept;
+// list[] receives the kept values in thread order.
+__device__ __forceinline__ int compact_ordered(bool keep, int value, int *list, int *wsum) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(keep);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kCoreThreads / 64; i++) {
+        const int c = wsum[i];
+        off += (i < w) ? c : 0;
+        tot += c;
+    }
+    if (keep) list[off + before] = value;
+    __syncthreads();
+    return tot;
+}
+
+__global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, Frc f, const unsigned char *cc_valid,
+                                                      int nchunk, const double *cc_slots, double *Ecore, double *Ecc,
+                                                      int terms) {
+    __shared__ int s_list[kCoreThreads];
+    __shared__ double s_pos[3 * kCoreThreads];
+    __shared__ int s_wsum[kCoreThreads / 64];
+    __shared__ double red[3 * kMaxRT * (kCoreThreads / 64)];
     const int kc = blockIdx.x;
     const int cb = s.core_bead[kc];
     const int tid = threadIdx.x;
     const double3 rc = ld3(g.r + 3 * cb), ac = ld3(g.a + 3 * cb), bc = ld3(g.b + 3 * cb), cc = ld3(g.c + 3 * cb);
     double3 F = zero3(), T = zero3();
-    double E = 0.0, Ed = 0.0;
+    double E = 0.0, Ed = 0.0, Ec = 0.0;
     const double *Qc = st.Q + 3 * (size_t)kc * s.Nq;
     const int lho = s.lh_of_core[kc];
 
     // --- charge-owned terms: thread per charge q of this core ---
-    for (int q = tid; q < s.Nq; q += blockDim.x) {
-        const double3 Qq = ld3(Qc + 3 * q);
-        const double cq = s.cq[q];
-        double3 Fq = zero3();
-        if ((terms & TERM_ELEC) && s.elig[cb]) {
-            // core-linker (func.cpp:1356-1487): force on the charge
-            for (int pi = 0; pi < s.n_lnk; pi++) {
-                int L = s.lnk[pi];
-                if (!s.elig[L]) continue;
-                double3 rl = ld3(g.r + 3 * L);
-                double3 dc = rc - rl;
-                if (sqrt(dot(dc, dc)) > 25.0) continue;
-                double3 z = Qq - rl;
-                double d = sqrt(dot(z, z));
-                double g1 = 0.0;
-                if (abs(cb - L) > 1) g1 += dh_core(s.k_e * s.q_l * cq, s.debye, d, E);
-                if (d <= 8.0 && cq > 0) g1 += ev_core(s.k_ex, s.sig_dc, d, E);
-                Fq += (g1 / d) * z;
+    double3 Fq[kCoreQPer];
+#pragma unroll
+    for (int i = 0; i < kCoreQPer; i++) Fq[i] = zero3();
+    if ((terms & TERM_ELEC) && s.elig[cb]) {
+        // core-linker (func.cpp:1356-1487): force on the charge
+        for (int base = 0; base < s.n_lnk; base += kCoreThreads) {
+            const int pi = base + tid;
+            bool keep = false;
+            int L = 0;
+            if (pi < s.n_lnk) {
+                L = s.lnk[pi];
+                if (s.elig[L]) {
+                    const double3 dc = rc - ld3(g.r + 3 * L);
+                    keep = !(sqrt(dot(dc, dc)) > 25.0);
+                }
             }
-        }
-        if (terms & TERM_LH) {
-            // LHG of other cores acting on this charge (func.cpp:2526-2586, charge side)
-            for (int k = 0; k < s.n_lhc; k++) {
-                int ob = s.lhc_bead[k];
-                if (ob == cb) continue;
-                double3 dc = ld3(g.r + 3 * ob) - rc;
-                if (sqrt(dot(dc, dc)) - s.rmax_g - s.rmax_q > 7.0 + 1e-6) continue;
-                for (int m = 0; m < s.n_lh_g; m++)
-                    dhev_recv(s.LH_q[m] * cq, s.c4pe, s.debye, s.k_ex, s.LH_hc[m], Qq,
-                              ld3(st.G + 3 * (k * s.n_lh_g + m)), Fq, Ed);
+            const int cnt = compact_ordered(keep, L, s_list, s_wsum);
+            for (int e = tid; e < cnt; e += kCoreThreads) st3(s_pos + 3 * e, ld3(g.r + 3 * s_list[e]));
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < kCoreQPer; i++) {
+                const int q = tid + i * kCoreThreads;
+                if (q < s.Nq) {
+                    const double3 Qq = ld3(Qc + 3 * q);
+                    const double cq = s.cq[q];
+                    for (int e = 0; e < cnt; e++) {
+                        const double3 z = Qq - ld3(s_pos + 3 * e);
+                        const double d = sqrt(dot(z, z));
+                        double g1 = 0.0;
+                        if (abs(cb - s_list[e]) > 1) g1 += dh_core(s.k_e * s.q_l * cq, s.debye, d, E);
+                        if (d <= 8.0 && cq > 0) g1 += ev_core(s.k_ex, s.sig_dc, d, E);
+                        Fq[i] += (g1 / d) * z;
+                    }
+                }
             }
+            __syncthreads();
         }
-        F += Fq;
-        T += torque_rel(Fq, ld3(s.cpos + 3 * q), ac, bc, cc);
+    }
+    if (terms & TERM_LH) {
+        // LHG of other cores acting on this charge (func.cpp:2526-2586, charge side)
+        for (int base = 0; base < s.n_lhc; base += kCoreThreads) {
+            const int k = base + tid;
+            bool keep = false;
+            if (k < s.n_lhc) {
+                const int ob = s.lhc_bead[k];
+                if (ob != cb) {
+                    const double3 dc = ld3(g.r + 3 * ob) - rc;
+                    keep = !(sqrt(dot(dc, dc)) - s.rmax_g - s.rmax_q > 7.0 + 1e-6);
+                }
+            }
+            const int cnt = compact_ordered(keep, k, s_list, s_wsum);
+#pragma unroll
+            for (int i = 0; i < kCoreQPer; i++) {
+                const int q = tid + i * kCoreThreads;
+                if (q < s.Nq) {
+                    const double3 Qq = ld3(Qc + 3 * q);
+                    const double cq = s.cq[q];
+                    for (int e = 0; e < cnt; e++) {
+                        const int k2 = s_list[e];
+                        for (int m = 0; m < s.n_lh_g; m++)
+                            dhev_recv(s.LH_q[m] * cq, s.c4pe, s.debye, s.k_ex, s.LH_hc[m], Qq,
+                                      ld3(st.G + 3 * (k2 * s.n_lh_g + m)), Fq[i], Ed);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kCoreQPer; i++) {
+        const int q = tid + i * kCoreThreads;
+        if (q < s.Nq) {
+            F += Fq[i];
+            T += torque_rel(Fq[i], ld3(s.cpos + 3 * q), ac, bc, cc);
+        }
     }
 
     // --- own globular-head sites (LH-bound cores) ---
     if ((terms & TERM_LH) && lho >= 0) {
         const int ng = s.n_lh_g;
         double3 Gm[kMaxGSites], Fm[kMaxGSites];
-        for (int m = 0; m < ng; m++) { Gm[m] = ld3(st.G + 3 * (lho * ng + m)); Fm[m] = zero3(); }
+        #pragma unroll
+        for (int m = 0; m < kMaxGSites; m++) if (m < ng) { Gm[m] = ld3(st.G + 3 * (lho * ng + m)); Fm[m] = zero3(); }
         // LHG-tail (2403-2476)
         for (int t = tid; t < s.n_tail; t += blockDim.x) {
             double3 rt = ld3(g.r_t + 3 * t);
-            for (int m = 0; m < ng; m++)
+#pragma unroll
+            for (int m = 0; m < kMaxGSites; m++) if (m < ng)
                 dhev_recv(s.LH_q[m] * s.t_q[t], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], Gm[m], rt, Fm[m], E);
         }
         // LHG-DNA (2481-2522)
         for (int pi = tid; pi < s.n_lnk; pi += blockDim.x) {
             double3 rl = ld3(g.r + 3 * s.lnk[pi]);
-            for (int m = 0; m < ng; m++)
+#pragma unroll
+            for (int m = 0; m < kMaxGSites; m++) if (m < ng)
                 dhev_recv(s.LH_q[m] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[m], Gm[m], rl, Fm[m], E);
         }
         // LHG-charges of every other core (2526-2586), culled per core
@@ -518,7 +595,8 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
             const double *Qk = st.Q + 3 * (size_t)k * s.Nq;
             for (int q = tid; q < s.Nq; q += blockDim.x) {
                 double3 Qq = ld3(Qk + 3 * q);
-                for (int m = 0; m < ng; m++)
+#pragma unroll
+                for (int m = 0; m < kMaxGSites; m++) if (m < ng)
                     dhev_recv(s.LH_q[m] * s.cq[q], s.c4pe, s.debye, s.k_ex, s.LH_hc[m], Gm[m], Qq, Fm[m], E);
             }
         }
@@ -528,7 +606,8 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
             int i2 = jl % s.nlh;
             i2 = (i2 < s.n_lh_n) ? i2 : i2 + s.n_lh_g;
             double3 rl = ld3(g.r_lh + 3 * jl);
-            for (int m = 0; m < ng; m++)
+#pragma unroll
+            for (int m = 0; m < kMaxGSites; m++) if (m < ng)
                 dhev_recv(s.LH_q[m] * s.LH_q[i2], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[i2]) / 2, Gm[m], rl, Fm[m], E);
         }
         // LHG-LHG with the other LH-bound cores (2649-2719)
@@ -536,14 +615,16 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
             int k = x / ng, m1 = x % ng;
             if (k == lho) continue;
             double3 G2 = ld3(st.G + 3 * x);
-            for (int m = 0; m < ng; m++)
+#pragma unroll
+            for (int m = 0; m < kMaxGSites; m++) if (m < ng)
                 dhev_recv(s.LH_q[m] * s.LH_q[m1], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[m1]) / 2, Gm[m], G2, Fm[m],
                           (k > lho) ? E : Ed);
         }
         // head - first C-term bead spring (2727-2777): force x10, torque x1
         if (tid == 0 && s.n_LH > 0) {
             double3 p = ld3(g.r_lh + 3 * (size_t)lho * s.nlh);
-            for (int m = 0; m < ng; m++) {
+            #pragma unroll
+        for (int m = 0; m < kMaxGSites; m++) if (m < ng) {
                 int ti = m + s.n_lh_n;
                 if (s.LH_conn[ti] != 1) continue;
                 double3 dv = p - Gm[m];
@@ -555,36 +636,58 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
                 T += torque_rel(make_double3(ks * S.x, ks * S.y, ks * S.z), ld3(s.LH_g_pos + 3 * m), ac, bc, cc);
             }
         }
-        for (int m = 0; m < ng; m++) {
+        #pragma unroll
+        for (int m = 0; m < kMaxGSites; m++) if (m < ng) {
             F += Fm[m];
             T += torque_rel(Fm[m], ld3(s.LH_g_pos + 3 * m), ac, bc, cc);
         }
     }
 
-    // --- anchored tails whose LH forces are redirected to this core ---
-    __shared__ double red[16 * 8];
+    // --- anchored tails whose LH forces are redirected to this core, up to
+    //     kMaxRT per pass over the LH partners ---
     if (terms & TERM_LH) {
-        for (int e = s.rt_off[kc]; e < s.rt_off[kc + 1]; e++) {
-            const int tj = s.rt_list[e];
-            const double3 rt = ld3(g.r_t + 3 * tj);
-            double3 Ft = zero3();
+        for (int e0 = s.rt_off[kc]; e0 < s.rt_off[kc + 1]; e0 += kMaxRT) {
+            const int ne = min(kMaxRT, s.rt_off[kc + 1] - e0);
+            double3 rt[kMaxRT], Ft[kMaxRT];
+            double tq[kMaxRT];
+#pragma unroll
+            for (int e = 0; e < kMaxRT; e++) {
+                const int tj = s.rt_list[e0 + min(e, ne - 1)];
+                rt[e] = ld3(g.r_t + 3 * tj);
+                tq[e] = s.t_q[tj];
+                Ft[e] = zero3();
+            }
             for (int i = tid; i < s.n_LH; i += blockDim.x) {
                 int idx = i % s.nlh;
                 idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
-                dhev_recv(s.LH_q[idx] * s.t_q[tj], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rt, ld3(g.r_lh + 3 * i), Ft, Ed);
+                const double3 rl = ld3(g.r_lh + 3 * i);
+#pragma unroll
+                for (int e = 0; e < kMaxRT; e++)
+                    if (e < ne) dhev_recv(s.LH_q[idx] * tq[e], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rt[e], rl, Ft[e], Ed);
             }
             for (int x = tid; x < s.n_lhc * s.n_lh_g; x += blockDim.x) {
-                int m = x % s.n_lh_g;
-                dhev_recv(s.LH_q[m] * s.t_q[tj], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], rt, ld3(st.G + 3 * x), Ft, Ed);
+                const int m = x % s.n_lh_g;
+                const double3 Gx = ld3(st.G + 3 * x);
+#pragma unroll
+                for (int e = 0; e < kMaxRT; e++)
+                    if (e < ne) dhev_recv(s.LH_q[m] * tq[e], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], rt[e], Gx, Ft[e], Ed);
             }
-            double fx = block_sum<kCoreThreads>(Ft.x, red);
-            double fy = block_sum<kCoreThreads>(Ft.y, red);
-            double fz = block_sum<kCoreThreads>(Ft.z, red);
-            if (tid == 0) {
-                double3 ff = make_double3(fx, fy, fz);
-                F += ff;
-                T += torque_abs(ff, rt, rc, ac, bc, cc);
+            double v[3 * kMaxRT];
+#pragma unroll
+            for (int e = 0; e < kMaxRT; e++) {
+                v[3 * e] = Ft[e].x;
+                v[3 * e + 1] = Ft[e].y;
+                v[3 * e + 2] = Ft[e].z;
             }
+            block_sum_n<kCoreThreads, 3 * kMaxRT>(v, red);
+            if (tid == 0)
+#pragma unroll
+                for (int e = 0; e < kMaxRT; e++)
+                    if (e < ne) {
+                        const double3 ff = make_double3(v[3 * e], v[3 * e + 1], v[3 * e + 2]);
+                        F += ff;
+                        T += torque_abs(ff, rt[e], rc, ac, bc, cc);
+                    }
         }
     }
     // --- anchor springs of this core's own tails (1971-2022); energy on the tail side ---
@@ -599,15 +702,37 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
             T += torque_rel(hs, ld3(s.tail_pos + 3 * jj), ac, bc, cc);
         }
     }
+    // --- core-core charge sums: this core's row (as c1) and column (as c2)
+    //     of the slot matrix (func.cpp:1489-1566) ---
+    if (terms & TERM_ELEC) {
+        double3 Tw = zero3();
+        for (int x = tid; x < s.n_c * nchunk; x += blockDim.x) {
+            const int o = x / nchunk, ch = x % nchunk;
+            if (o == kc) continue;
+            const bool first = kc < o;
+            const size_t pidx = first ? (size_t)kc * s.n_c + o : (size_t)o * s.n_c + kc;
+            if (!cc_valid[pidx]) continue;
+            const double *sl = cc_slots + 10 * (pidx * nchunk + ch);
+            if (first) {
+                F += make_double3(sl[0], sl[1], sl[2]);
+                T += make_double3(sl[3], sl[4], sl[5]);
+                Ec += sl[9];
+            } else {
+                F -= make_double3(sl[0], sl[1], sl[2]);
+                Tw += make_double3(sl[6], sl[7], sl[8]);
+            }
+        }
+        T += make_double3(dot(ac, Tw), dot(bc, Tw), dot(cc, Tw));
+    }
     // block reduction in fixed order
-    double v[7] = {F.x, F.y, F.z, T.x, T.y, T.z, E};
-    double out[7];
-    for (int q = 0; q < 7; q++) out[q] = block_sum<kCoreThreads>(v[q], red);
+    double v[8] = {F.x, F.y, F.z, T.x, T.y, T.z, E, Ec};
+    block_sum_n<kCoreThreads, 8>(v, red);
     if (tid == 0) {
         double *Fc = f.F + 3 * cb, *Tc = f.T + 3 * cb;
-        Fc[0] += out[0]; Fc[1] += out[1]; Fc[2] += out[2];
-        Tc[0] += out[3]; Tc[1] += out[4]; Tc[2] += out[5];
-        if (Epart) Epart[kc] = out[6];
+        Fc[0] += v[0]; Fc[1] += v[1]; Fc[2] += v[2];
+        Tc[0] += v[3]; Tc[1] += v[4]; Tc[2] += v[5];
+        if (Ecore) Ecore[kc] = v[6];
+        if (Ecc) Ecc[kc] = v[7];
     }
 }
 
@@ -644,14 +769,16 @@ constexpr int kCCChunkMax = 160;
 // Work unit = (pair, chunk of the second core's charges).  Thread = charge
 // of the first core.  Slot = {F on c1 (3), T on c1 body (3), T on c2 world
 // (3), E}.  The torque on c2 is summed in world axes about its centre and
-// turned into body components by k_cc_reduce (equal to the reference's
+// turned into body components by k_core (equal to the reference's
 // body-frame sum for an orthonormal frame).
+template <bool WITH_E>
 __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, Sites st, const int *list,
                                                        const int *count, int nchunk, double *slots) {
     __shared__ double sq[kCCChunkMax * 7];
-    __shared__ double red[16];
+    __shared__ double red[10 * (kCCThreads / 64)];
     const int nunits = (*count) * nchunk;
     const int chunk = (s.Nq + nchunk - 1) / nchunk;
+    const double kappa = s.debye, kex = s.k_ex, sg = s.sig_cc, c12 = 12.0 / sg, c6 = 6.0 / sg;
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int pr = u / nchunk, ch = u % nchunk;
         const int k1 = list[2 * pr], k2 = list[2 * pr + 1];
@@ -675,11 +802,21 @@ __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, Sites st,
             double3 Fk = zero3();
             for (int l = 0; l < l1 - l0; l++) {
                 const double *d = sq + 7 * l;
-                double3 z = make_double3(Qk.x - d[0], Qk.y - d[1], Qk.z - d[2]);
-                double dist = sqrt(dot(z, z));
-                double g1 = dh_core(keq * d[3], s.debye, dist, E);
-                if (dist <= 8.0) g1 += ev_core(s.k_ex, s.sig_cc, dist, E);
-                double3 fk = (g1 / dist) * z;
+                const double3 z = make_double3(Qk.x - d[0], Qk.y - d[1], Qk.z - d[2]);
+                // Debye-Hueckel + excluded volume of the reference (dh_core, ev_core)
+                // with 1/d from a refined reciprocal square root
+                const double r2s = dot(z, z);
+                const double inv = rsqrt_f64(r2s);
+                const double dist = r2s * inv;
+                const double t = keq * d[3] * exp(-kappa * dist);
+                if (WITH_E) E += t * inv;
+                double g1d = t * (kappa * dist + 1.0) * (inv * inv * inv);
+                if (dist <= 8.0) {
+                    const double x = sg * inv, x2 = x * x, x6 = x2 * x2 * x2, x7 = x6 * x, x13 = x6 * x7;
+                    if (WITH_E) E += kex * (x6 * x6 - x6);
+                    g1d += kex * (c12 * x13 - c6 * x7) * inv;
+                }
+                const double3 fk = g1d * z;
                 Fk += fk;
                 // torque on c2 about its centre from -fk at lever (Q_l - r2)
                 T2.x -= d[5] * fk.z - d[6] * fk.y;
@@ -690,46 +827,11 @@ __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, Sites st,
             T1 += torque_rel(Fk, ld3(s.cpos + 3 * k), a1, bb1, c1);
         }
         double v[10] = {F.x, F.y, F.z, T1.x, T1.y, T1.z, T2.x, T2.y, T2.z, E};
+        block_sum_n<kCCThreads, 10>(v, red);
         double *slot = slots + 10 * ((size_t)(k1 * s.n_c + k2) * nchunk + ch);
-        for (int q = 0; q < 10; q++) {
-            double t = block_sum<kCCThreads>(v[q], red);
-            if (threadIdx.x == 0) slot[q] = t;
-        }
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 10; q++) slot[q] = v[q];
     }
-}
-
-// Thread per core: gathers its row (as c1) and column (as c2) of the slot
-// matrix in index order.  F +=, T +=.  Energy owner: the lower index.
-__global__ void k_cc_reduce(Stat s, Geo g, const unsigned char *valid, int nchunk, const double *slots, Frc f,
-                            double *Epart) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= s.n_c) return;
-    const int cb = s.core_bead[k];
-    double3 F = zero3(), T = zero3(), Tw = zero3();
-    double E = 0.0;
-    for (int o = 0; o < s.n_c; o++) {
-        if (o == k) continue;
-        const bool first = k < o;
-        const size_t pidx = first ? (size_t)k * s.n_c + o : (size_t)o * s.n_c + k;
-        if (!valid[pidx]) continue;
-        for (int ch = 0; ch < nchunk; ch++) {
-            const double *sl = slots + 10 * (pidx * nchunk + ch);
-            if (first) {
-                F += make_double3(sl[0], sl[1], sl[2]);
-                T += make_double3(sl[3], sl[4], sl[5]);
-                E += sl[9];
-            } else {
-                F -= make_double3(sl[0], sl[1], sl[2]);
-                Tw += make_double3(sl[6], sl[7], sl[8]);
-            }
-        }
-    }
-    const double3 a = ld3(g.a + 3 * cb), b = ld3(g.b + 3 * cb), c = ld3(g.c + 3 * cb);
-    T += make_double3(dot(a, Tw), dot(b, Tw), dot(c, Tw));
-    double *Fc = f.F + 3 * cb, *Tc = f.T + 3 * cb;
-    Fc[0] += F.x; Fc[1] += F.y; Fc[2] += F.z;
-    Tc[0] += T.x; Tc[1] += T.y; Tc[2] += T.z;
-    if (Epart) Epart[k] = E;
 }
 
 // Energy partials -> one double, fixed order (one workgroup).
@@ -771,7 +873,6 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
     if (s.n_lnk) { M(K_LINKER, 0); k_linker<<<s.n_lnk, kRecvThreads, 0, q>>>(s, g, st, f, E(off_lnk), terms); M(K_LINKER, 1); }
     if (s.n_tail) { M(K_TAIL, 0); k_tail<<<s.n_tail, kRecvThreads, 0, q>>>(s, g, st, f, E(off_tail), terms); M(K_TAIL, 1); }
     if (s.n_LH) { M(K_LH, 0); k_lh<<<s.n_LH, kRecvThreads, 0, q>>>(s, g, st, f, E(off_lh), terms); M(K_LH, 1); }
-    M(K_CORE, 0); k_core<<<s.n_c, kCoreThreads, 0, q>>>(s, g, st, f, E(off_core), terms); M(K_CORE, 1);
     if (terms & TERM_ELEC) {
         long long nn = (long long)s.n_c * s.n_c;
         M(K_CCLIST, 0);
@@ -779,14 +880,15 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
         k_cc_list<<<nblk(nn, 256), 256, 0, q>>>(s, g, w.cc_valid, w.cc_list, w.cc_count);
         M(K_CCLIST, 1);
         M(K_CCWORK, 0);
-        k_cc_work<<<w.cc_grid, kCCThreads, 0, q>>>(s, g, st, w.cc_list, w.cc_count, w.cc_nchunk, w.cc_slots);
+        if (Epart)
+            k_cc_work<true><<<w.cc_grid, kCCThreads, 0, q>>>(s, g, st, w.cc_list, w.cc_count, w.cc_nchunk, w.cc_slots);
+        else
+            k_cc_work<false><<<w.cc_grid, kCCThreads, 0, q>>>(s, g, st, w.cc_list, w.cc_count, w.cc_nchunk, w.cc_slots);
         M(K_CCWORK, 1);
-        M(K_CCRED, 0);
-        k_cc_reduce<<<nblk(s.n_c, 64), 64, 0, q>>>(s, g, w.cc_valid, w.cc_nchunk, w.cc_slots, f, E(off_cc));
-        M(K_CCRED, 1);
-    } else if (Epart) {
-        k_fill<<<nblk(s.n_c, 64), 64, 0, q>>>(Epart + off_cc, s.n_c, 0.0);
     }
+    M(K_CORE, 0);
+    k_core<<<s.n_c, kCoreThreads, 0, q>>>(s, g, st, f, w.cc_valid, w.cc_nchunk, w.cc_slots, E(off_core), E(off_cc), terms);
+    M(K_CORE, 1);
 }
 
 int energy_slots(const Stat &s) { return s.n + s.n_lnk + s.n_tail + s.n_LH + 2 * s.n_c; }
