@@ -7,13 +7,12 @@
 //
 // Matrix: row-major, lower triangle, padded to a multiple of 64 with an
 // identity block (so the padded factor is [[L,0],[0,I]] and no kernel needs
-// a bounds test).  Right-looking blocked algorithm with 64x64 tiles, two
-// launches per block column kb:
-//   trsm   L_ib,kb L_kb,kb^T = A_ib,kb           (substitution, a wavefront
-//          per row tile, FP64)
-//   syrk   A_ib,jb -= L_ib,kb * L_jb,kb^T        (FP64 or FP32 matrix cores)
-//          and the workgroup owning tile (kb+1,kb+1) factors it right after
-//          its update (one wavefront, FP64).
+// a bounds test).  Right-looking blocked algorithm with 64x64 tiles, one
+// launch per block column kb (k_step): every workgroup solves the two
+// panel tiles it needs (substitution, a wavefront each, FP64), updates its
+// trailing tile A_ib,jb -= L_ib,kb L_jb,kb^T (FP64 or FP32 matrix cores),
+// and the workgroup owning tile (kb+1,kb+1) then factors it (one
+// wavefront, FP64).
 #include "bd_dev.h"
 #include "bd_kernels.h"
 
@@ -285,44 +284,49 @@ __global__ void __launch_bounds__(256) k_potrf0(double *A, int ld, double *Lp, d
     if ((threadIdx.x >> 6) == 0) diag_factor(T, LP, A, ld, 0, Lp, invd, neg);
 }
 
-// Panel below the diagonal: X L_kb,kb^T = A_ib,kb for the row tiles
-// ib > kb, one tile per wavefront (four per workgroup), one row per lane,
-// by substitution against the diagonal factor in panel order.
-__global__ void __launch_bounds__(256) k_trsm(double *A, int ld, int kb, const double *Lp, const double *invd,
-                                              int nrow_blocks) {
-    __shared__ double LP[TB * TB], iv[TB];
-    const int tid = threadIdx.x, w = tid >> 6;
-    for (int e = tid; e < TB * TB / 2; e += blockDim.x)
-        reinterpret_cast<double2 *>(LP)[e] = reinterpret_cast<const double2 *>(Lp + (size_t)kb * TB * TB)[e];
-    if (tid < TB) iv[tid] = invd[kb * TB + tid];
-    __syncthreads();
-    const int t = blockIdx.x * 4 + w;
-    if (t >= nrow_blocks) return;
-    const int ib = kb + 1 + t;
-    double b[TB];
-    load_row(A, ld, ib * TB, kb * TB, b);
-    solve_panel8<0>(b, LP, iv);
-    solve_panel8<8>(b, LP, iv);
-    solve_panel8<16>(b, LP, iv);
-    solve_panel8<24>(b, LP, iv);
-    solve_panel8<32>(b, LP, iv);
-    solve_panel8<40>(b, LP, iv);
-    solve_panel8<48>(b, LP, iv);
-    solve_panel8<56>(b, LP, iv);
-    store_row(A, ld, ib * TB, kb * TB, b, false);
-}
-
-// Trailing update of tile (ib, jb), kb < jb <= ib; the workgroup of tile
-// (kb+1, kb+1) then factors it (wavefront 0).  FP32 matrix
-// cores with FP32 accumulation when MODE != 0.
+// One block column kb of the factorisation in one launch, one workgroup per
+// trailing lower tile (ib, jb), kb < jb <= ib:
+//   1. the panel tiles L_ib,kb and L_jb,kb are solved from A against the
+//      diagonal factor of step kb (wavefronts 0 and 1, row per lane,
+//      X L_kb,kb^T = A); the workgroup of the diagonal tile (ib, ib)
+//      stores L_ib,kb, so every panel tile is written exactly once, into the
+//      upper block (kb, ib) (k_panel_copy moves them after the last step);
+//   2. A_ib,jb -= L_ib,kb L_jb,kb^T on the matrix cores (FP64, or FP32 with
+//      FP32 accumulation when MODE != 0);
+//   3. the workgroup of tile (kb+1, kb+1) factors it (wavefront 0), which
+//      the next launch needs.
+// The panel solves are repeated by the workgroups of a block row, in
+// exchange for one launch (and one kernel boundary) per block column.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_syrk(double *A, int ld, int kb, double *Lp, double *invd, int *neg) {
-    __shared__ double As[TB * LDP], Bs[TB * LDP];
+__global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, double *Lp, double *invd, int *neg) {
+    __shared__ double LPs[TB * TB], iv[TB], As[TB * LDP], Bs[TB * LDP];
     int ii, jj;
     tri_index(blockIdx.x, ii, jj);
     const int ib = kb + 1 + ii, jb = kb + 1 + jj;
-    load_tile(A, ld, ib * TB, kb * TB, As);
-    if (ib != jb) load_tile(A, ld, jb * TB, kb * TB, Bs);
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    for (int e = tid; e < TB * TB / 2; e += blockDim.x)
+        reinterpret_cast<double2 *>(LPs)[e] = reinterpret_cast<const double2 *>(Lp + (size_t)kb * TB * TB)[e];
+    if (tid < TB) iv[tid] = invd[kb * TB + tid];
+    __syncthreads();
+    if (w == 0 || (w == 1 && ib != jb)) {
+        const int rb = (w == 0) ? ib : jb;
+        double b[TB];
+        load_row(A, ld, rb * TB, kb * TB, b);
+        solve_panel8<0>(b, LPs, iv);
+        solve_panel8<8>(b, LPs, iv);
+        solve_panel8<16>(b, LPs, iv);
+        solve_panel8<24>(b, LPs, iv);
+        solve_panel8<32>(b, LPs, iv);
+        solve_panel8<40>(b, LPs, iv);
+        solve_panel8<48>(b, LPs, iv);
+        solve_panel8<56>(b, LPs, iv);
+        double *S = (w == 0) ? As : Bs;
+#pragma unroll
+        for (int c = 0; c < TB; c++) S[lane * LDP + c] = b[c];
+        // the other workgroups of this launch still read A_ib,kb: the solved
+        // tile goes to the unused upper block (kb, ib), moved by k_panel_copy
+        if (w == 0 && ib == jb) store_row(A, ld, kb * TB, ib * TB, b, false);
+    }
     __syncthreads();
     const double *Bp = (ib != jb) ? Bs : As;
     if (MODE == 0) {
@@ -346,7 +350,7 @@ __global__ void __launch_bounds__(256) k_syrk(double *A, int ld, int kb, double 
     }
     __syncthreads();
     if (ii == 0 && jj == 0) {
-        if ((threadIdx.x >> 6) == 0) diag_factor(As, Bs, A, ld, ib * TB, Lp + (size_t)ib * TB * TB, invd, neg);
+        if (w == 0) diag_factor(As, Bs, A, ld, ib * TB, Lp + (size_t)ib * TB * TB, invd, neg);
     } else {
         store_tile(A, ld, ib * TB, jb * TB, As, ib == jb);
     }
@@ -378,6 +382,18 @@ void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, co
     (void)n_hi;
 }
 
+// Moves every solved panel tile from its upper block (kb, ib) to (ib, kb).
+__global__ void __launch_bounds__(256) k_panel_copy(double *A, int ld) {
+    int ii, jj;
+    tri_index(blockIdx.x, ii, jj);
+    const int ib = ii + 1, kb = jj;
+    for (int e = threadIdx.x; e < TB * TB / 2; e += blockDim.x) {
+        const int r = e / (TB / 2), c = 2 * (e % (TB / 2));
+        *reinterpret_cast<double2 *>(A + (size_t)(ib * TB + r) * ld + kb * TB + c) =
+            *reinterpret_cast<const double2 *>(A + (size_t)(kb * TB + r) * ld + ib * TB + c);
+    }
+}
+
 void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_arg) {
     const int nt = c.mpad / TB;
     auto M_ = [&](int id, int ph) { if (mark) mark(mark_arg, id, ph); };
@@ -387,15 +403,17 @@ void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_a
     M_(K_POTRF, 1);
     for (int kb = 0; kb + 1 < nt; kb++) {
         const int M = nt - kb - 1;
-        M_(K_TRSM, 0);
-        k_trsm<<<(M + 3) / 4, 256, 0, q>>>(c.A, c.ld, kb, c.Lp, c.invd, M);
-        M_(K_TRSM, 1);
         M_(K_SYRK, 0);
         if (c.mode == 0)
-            k_syrk<0><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Lp, c.invd, c.neg);
+            k_step<0><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Lp, c.invd, c.neg);
         else
-            k_syrk<1><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Lp, c.invd, c.neg);
+            k_step<1><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Lp, c.invd, c.neg);
         M_(K_SYRK, 1);
+    }
+    if (nt > 1) {
+        M_(K_TRSM, 0);
+        k_panel_copy<<<nt * (nt - 1) / 2, 256, 0, q>>>(c.A, c.ld);
+        M_(K_TRSM, 1);
     }
 }
 
