@@ -166,6 +166,41 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 // update of a row reads the 8 panel entries of column c as 4 broadcast
 // 16-byte LDS loads and applies 8 FMAs.
 // ---------------------------------------------------------------------------
+// Rank-8 update of the columns c >= C0 of a row held in a[]: a[c] -=
+// sum_q x[q] * lp[c*8 + q].  The 8 panel entries of column c are 4
+// broadcast 16-byte LDS loads, issued two columns ahead of their FMAs (a
+// scheduling barrier per column keeps that distance; left to itself the
+// scheduler reuses one register quad and exposes every LDS latency).
+template <int C0>
+__device__ __forceinline__ void rank8_update(double (&a)[TB], const double (&x)[8], const double *lp) {
+    const double2 *l2 = reinterpret_cast<const double2 *>(lp);
+    double2 p0[4], p1[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) p0[k] = (C0 < TB) ? l2[C0 * 4 + k] : double2{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 4; k++) p1[k] = (C0 + 1 < TB) ? l2[(C0 + 1) * 4 + k] : double2{0.0, 0.0};
+#pragma unroll
+    for (int c = C0; c < TB; c++) {
+        double2 cur[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            cur[k] = p0[k];
+            p0[k] = p1[k];
+        }
+        if (c + 2 < TB)
+#pragma unroll
+            for (int k = 0; k < 4; k++) p1[k] = l2[(c + 2) * 4 + k];
+        double acc = a[c];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            acc -= x[2 * k] * cur[k].x;
+            acc -= x[2 * k + 1] * cur[k].y;
+        }
+        a[c] = acc;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // Panel P0..P0+7 of the factor of a 64x64 tile held one row per lane in
 // a[] (entries right of the diagonal are ignored): the CPU algorithm of
 // func.cpp:806-836 in right-looking order -- pivot f (|f| and a count if
@@ -196,13 +231,11 @@ __device__ __forceinline__ void factor_panel8(double (&a)[TB], double *LP, doubl
         v.y = a[P0 + q + 1];
         *reinterpret_cast<double2 *>(lp + lane * 8 + q) = v;
     }
+    double x[8];
 #pragma unroll
-    for (int c = P0 + 8; c < TB; c++) {
-        double acc = a[c];
-#pragma unroll
-        for (int q = 0; q < 8; q++) acc -= a[P0 + q] * lp[c * 8 + q];
-        a[c] = acc;
-    }
+    for (int q = 0; q < 8; q++) x[q] = a[P0 + q];
+    __builtin_amdgcn_sched_barrier(0);
+    rank8_update<P0 + 8>(a, x, lp);
 }
 
 // Panel P0..P0+7 of the row solve x L^T = b (row per lane in b[]) with the
@@ -217,13 +250,11 @@ __device__ __forceinline__ void solve_panel8(double (&b)[TB], const double *LP, 
 #pragma unroll
         for (int c = j + 1; c < P0 + 8; c++) b[c] -= xj * lp[c * 8 + (j - P0)];
     }
+    double x[8];
 #pragma unroll
-    for (int c = P0 + 8; c < TB; c++) {
-        double acc = b[c];
-#pragma unroll
-        for (int q = 0; q < 8; q++) acc -= b[P0 + q] * lp[c * 8 + q];
-        b[c] = acc;
-    }
+    for (int q = 0; q < 8; q++) x[q] = b[P0 + q];
+    __builtin_amdgcn_sched_barrier(0);
+    rank8_update<P0 + 8>(b, x, lp);
 }
 
 __device__ __forceinline__ void load_row(const double *A, int ld, int r0, int c0, double (&b)[TB]) {
