@@ -85,16 +85,6 @@ def chol_flops(m):
     return m ** 3 / 3.0
 
 
-def syrk_flops_total(m, tb=64):
-    """FLOPs of the trailing updates of the tiled factor (2 per MAC)."""
-    nt = (m + tb - 1) // tb
-    tot = 0.0
-    for kb in range(nt - 1):
-        M = nt - kb - 1
-        tot += M * (M + 1) / 2 * 2.0 * tb ** 3
-    return tot
-
-
 def main():
     args = parse()
     if args.gpus != 1:
@@ -161,13 +151,12 @@ def main():
     dominant = max(per_step, key=per_step.get)
 
     # roofline of the dominant kernel
-    if dominant in ("chol", "syrk", "trsm", "potrf"):
-        # the factorisation's algorithmic FLOPs (n^3/3 per factor; the
-        # trailing-update share when the per-panel-step kernels are used)
-        flops = syrk_flops_total(n_hi3) if dominant == "syrk" else chol_flops(n_hi3)
+    if dominant in ("chol_step", "potrf"):
+        # the factorisation's algorithmic FLOPs, n^3/3, spread over the
+        # block-column launches (one per 64 columns after the first)
         tot_ms, launches = kt[dominant]
         avg_s = tot_ms / launches / 1e3
-        per_launch = flops / (launches / args.timing_steps)
+        per_launch = chol_flops(n_hi3) / (launches / args.timing_steps)
         roof = dict(bound="mfma", achieved=per_launch / avg_s / 1e12, peak=PEAK_FP64 / 1e12,
                     unit="TFLOP/s", kernel=dominant, pipe="FP64 MFMA",
                     flop_per_launch=per_launch, avg_launch_us=avg_s * 1e6)

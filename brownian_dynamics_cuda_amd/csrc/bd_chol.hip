@@ -434,17 +434,17 @@ void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_a
     M_(K_POTRF, 1);
     for (int kb = 0; kb + 1 < nt; kb++) {
         const int M = nt - kb - 1;
-        M_(K_SYRK, 0);
+        M_(K_CHOL_STEP, 0);
         if (c.mode == 0)
             k_step<0><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Lp, c.invd, c.neg);
         else
             k_step<1><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Lp, c.invd, c.neg);
-        M_(K_SYRK, 1);
+        M_(K_CHOL_STEP, 1);
     }
     if (nt > 1) {
-        M_(K_TRSM, 0);
+        M_(K_PANEL_COPY, 0);
         k_panel_copy<<<nt * (nt - 1) / 2, 256, 0, q>>>(c.A, c.ld);
-        M_(K_TRSM, 1);
+        M_(K_PANEL_COPY, 1);
     }
 }
 

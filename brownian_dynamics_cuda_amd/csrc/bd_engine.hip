@@ -54,7 +54,7 @@ int fail(int code, const char *fmt, ...) {
 
 const char *kKernelNames[K_COUNT] = {
     "sites", "mech", "linker", "tail", "lh", "core", "cc_list", "cc_work", "rpy",
-    "potrf", "trsm", "syrk", "noise", "frames", "betas", "rotate", "translate", "average",
+    "potrf", "panel_copy", "chol_step", "noise", "frames", "betas", "rotate", "translate", "average",
     "commit", "energy_reduce"};
 
 }  // namespace
