@@ -149,3 +149,18 @@ def test_engine_requires_library_and_fails_loudly(monkeypatch):
     monkeypatch.setattr(E, "_LIB", None)
     with pytest.raises(RuntimeError):
         E.load_library("/nonexistent/libbdhip.so")
+
+
+def test_reference_deck_files_match_the_packed_fixture():
+    """The raw input files of the reference deck (tests/golden/testdata) read
+    by system.py give the arrays packed in testdata_deck.npz."""
+    import os
+    from brownian_dynamics_cuda_amd import system as S
+    from decks import GOLDEN, load_golden
+    fib, st = S.load_reference_deck(os.path.join(GOLDEN, "testdata"))
+    fib2, st2, _ = load_golden("testdata_deck")
+    for k in ("r", "a", "b", "c", "r_t", "r_lh"):
+        assert np.array_equal(getattr(st, k), getattr(st2, k)), k
+    for k in ("core_pos", "core_q", "t_q", "h_t", "LH_q", "phi_o", "nc_lh_flag"):
+        assert np.array_equal(np.asarray(getattr(fib, k)), np.asarray(getattr(fib2, k))), k
+    assert fib.phys.debye == fib2.phys.debye and fib.phys.q_l == fib2.phys.q_l
