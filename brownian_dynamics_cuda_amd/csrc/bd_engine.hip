@@ -82,7 +82,17 @@ struct bd_context {
     int ne = 0;
     bool state_set = false;
     bool use_graph = true;
-    hipGraphExec_t graph = nullptr;
+    // captured steps: [0] with D and its factor, [1] reusing the factor
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // frequency_RP (main.cpp:45): D and L are rebuilt on the first step after
+    // an invalidation (new state, new hydrodynamic size, precision change)
+    // and then every rp_freq-th step; the other steps reuse L for rd
+    int rp_freq = 1;
+    long long rp_count = 0;
+    bool factor_fresh = false;
+    // noise and energies of bd_run_steps, grown on demand
+    double *batch = nullptr;
+    size_t batch_cap = 0;
     // timing
     bool timing = false;
     std::vector<std::pair<int, hipEvent_t>> pending;  // (kernel id, event) pairs, start/stop
@@ -168,8 +178,18 @@ void enqueue_factor(bd_context *c, const Geo &g, hipStream_t q) {
     launch_cholesky(c->chol, q, M.fn(), c);
 }
 
+void drop_graphs(bd_context *c) {
+    for (auto &g : c->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
 // One predictor-corrector step (func_cuda.cu:3494-3660) reading c->rr, c->p.
-void enqueue_step(bd_context *c) {
+// refactor=false keeps the L of an earlier step (the frequency_RP gate of
+// func_cuda.cu:3504-3535) and only recomputes rd = s2dt L p.
+void enqueue_step(bd_context *c, bool refactor) {
     Marker M{c};
     hipStream_t q = c->stream;
     const Stat &s = c->st;
@@ -181,7 +201,7 @@ void enqueue_step(bd_context *c) {
         (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
         qf = c->side;
     }
-    enqueue_factor(c, c->cur, qf);
+    if (refactor) enqueue_factor(c, c->cur, qf);
     M(K_NOISE, 0); launch_noise(s, c->n_hi, c->n_D3, c->chol, c->p, c->rad_all, c->rd, qf); M(K_NOISE, 1);
     // LH bending angles at the current state (3497)
     M(K_BETAS, 0); launch_betas(s, c->cur, 0, 1, q); M(K_BETAS, 1);
@@ -474,10 +494,11 @@ int bd_destroy(bd_context *c) {
     if (!c) return BD_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    drop_graphs(c);
     for (auto &pe : c->pending) (void)hipEventDestroy(pe.second);
     if (c->side) (void)hipStreamSynchronize(c->side);
     for (void *p : c->allocs) (void)hipFree(p);
+    if (c->batch) (void)hipFree(c->batch);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->side) (void)hipStreamDestroy(c->side);
@@ -513,6 +534,7 @@ int bd_set_lh(bd_context *c, const int *nc_lh_flag, int n_LH, const double *r_lh
         e |= h2d(c->rad_all + c->n + c->n_tail, lh_rad, n_LH);
         return e ? BD_E_HIP : BD_OK;
     }
+    const int n_hi_old = c->n_hi;
     lhc.resize(n_c, 0);
     HIPCHK(hipMemcpy((void *)c->st.lhc_bead, lhc.data(), sizeof(int) * n_c, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy((void *)c->st.lh_of_core, loc.data(), sizeof(int) * n_c, hipMemcpyHostToDevice));
@@ -531,9 +553,10 @@ int bd_set_lh(bd_context *c, const int *nc_lh_flag, int n_LH, const double *r_lh
     c->chol.m = 3 * c->n_hi;
     c->chol.mpad = ((c->chol.m + 63) / 64) * 64;
     c->ne = energy_slots(c->st);
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
+    drop_graphs(c);
+    if (c->n_hi != n_hi_old) {
+        c->factor_fresh = false;
+        c->rp_count = 0;
     }
     if (n_LH) launch_betas(c->st, c->cur, 0, 1, c->stream);
     return sync_check(c);
@@ -542,10 +565,9 @@ int bd_set_lh(bd_context *c, const int *nc_lh_flag, int n_LH, const double *r_lh
 int bd_set_integrator(bd_context *c, double a1, double a2, double s2dt, double del, double time_step) {
     if (check(c)) return BD_E_ARG;
     c->st.a1 = a1; c->st.a2 = a2; c->st.s2dt = s2dt; c->st.del = del; c->st.time_step = time_step;
-    if (c->graph) {
+    if (c->graph[0] || c->graph[1]) {
         HIPCHK(hipStreamSynchronize(c->stream));
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
+        drop_graphs(c);
     }
     return BD_OK;
 }
@@ -567,6 +589,8 @@ int bd_set_state(bd_context *c, const double *r, const double *a, const double *
         launch_betas(c->st, c->cur, 1, 1, c->stream);
     }
     c->state_set = true;
+    c->factor_fresh = false;
+    c->rp_count = 0;
     return sync_check(c);
 }
 
@@ -609,23 +633,27 @@ int bd_get_derived(bd_context *c, bd_derived *d) {
 }
 
 static int run_step(bd_context *c) {
+    const bool refactor = !c->factor_fresh || c->rp_count % c->rp_freq == 0;
+    c->rp_count++;
+    c->factor_fresh = true;
     if (c->timing || !c->use_graph) {
-        enqueue_step(c);
+        enqueue_step(c, refactor);
         HIPCHK(hipGetLastError());
         if (c->timing) collect_timing(c);
         return BD_OK;
     }
-    if (!c->graph) {
+    hipGraphExec_t &ge = c->graph[refactor ? 0 : 1];
+    if (!ge) {
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_step(c);
+        enqueue_step(c, refactor);
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (e != hipSuccess) return fail(BD_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-        e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return fail(BD_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
     }
-    HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    HIPCHK(hipGraphLaunch(ge, c->stream));
     return BD_OK;
 }
 
@@ -654,6 +682,37 @@ int bd_step_device(bd_context *c, const double *d_rr, const double *d_p, double 
     if (d_energy)
         HIPCHK(hipMemcpyAsync(d_energy, c->energy, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     return BD_OK;
+}
+
+int bd_run_steps(bd_context *c, int k, const double *rr, const double *p, double *energies) {
+    if (check(c)) return BD_E_ARG;
+    if (!c->state_set) return fail(BD_E_STATE, "bd_set_state must be called before bd_run_steps");
+    if (k < 0 || (k && (!rr || !p))) return fail(BD_E_ARG, "k >= 0 and rr, p are required");
+    if (!k) return BD_OK;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n3 = 3 * (size_t)c->n, nd3 = (size_t)c->n_D3;
+    const size_t need = (size_t)k * (n3 + nd3 + 1);
+    if (need > c->batch_cap) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->batch) (void)hipFree(c->batch);
+        c->batch = nullptr;
+        c->batch_cap = 0;
+        hipError_t e = hipMalloc((void **)&c->batch, need * sizeof(double));
+        if (e != hipSuccess) return fail(BD_E_HIP, "hipMalloc(%zu): %s", need * sizeof(double), hipGetErrorString(e));
+        c->batch_cap = need;
+    }
+    double *d_rr = c->batch, *d_p = d_rr + (size_t)k * n3, *d_e = d_p + (size_t)k * nd3;
+    HIPCHK(hipMemcpyAsync(d_rr, rr, sizeof(double) * k * n3, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_p, p, sizeof(double) * k * nd3, hipMemcpyHostToDevice, c->stream));
+    for (int s = 0; s < k; s++) {
+        HIPCHK(hipMemcpyAsync(c->rr, d_rr + s * n3, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->p, d_p + s * nd3, sizeof(double) * nd3, hipMemcpyDeviceToDevice, c->stream));
+        int e = run_step(c);
+        if (e) return e;
+        HIPCHK(hipMemcpyAsync(d_e + s, c->energy, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (energies) HIPCHK(hipMemcpyAsync(energies, d_e, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
+    return sync_check(c);
 }
 
 int bd_forces(bd_context *c, int terms, double *F, double *T, double *tF, double *lhF, double *energy) {
@@ -703,6 +762,7 @@ int bd_get_tensor(bd_context *c, double *D) {
     if (!c->state_set) return fail(BD_E_STATE, "bd_set_state must be called first");
     HIPCHK(hipSetDevice(c->device));
     launch_rpy(c->st, c->n_hi, c->cur, c->rad_all, c->chol, c->stream);
+    c->factor_fresh = false;  // D now overwrites the factor's storage
     int e = sync_check(c);
     if (e) return e;
     const int m = c->chol.m;
@@ -726,12 +786,22 @@ int bd_noise_displacement(bd_context *c, const double *p, double *rd) {
 int bd_set_chol_mode(bd_context *c, int mode) {
     if (check(c)) return BD_E_ARG;
     if (mode != BD_CHOL_FP64 && mode != BD_CHOL_FP32) return fail(BD_E_ARG, "unsupported chol_mode %d", mode);
-    if (mode != c->chol.mode && c->graph) {
+    if (mode != c->chol.mode) {
         HIPCHK(hipStreamSynchronize(c->stream));
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
+        drop_graphs(c);
+        c->factor_fresh = false;
+        c->rp_count = 0;
     }
     c->chol.mode = mode;
+    return BD_OK;
+}
+
+int bd_set_frequency_rp(bd_context *c, int frequency_rp) {
+    if (check(c)) return BD_E_ARG;
+    if (frequency_rp < 1) return fail(BD_E_ARG, "frequency_RP must be >= 1 (got %d)", frequency_rp);
+    c->rp_freq = frequency_rp;
+    c->rp_count = 0;
+    c->factor_fresh = false;
     return BD_OK;
 }
 

@@ -22,6 +22,7 @@ struct Legacy {
     std::vector<int> flags;
     int n_LH = -1;
     double a1 = 0, a2 = 0, s2dt = 0, del = 0, dt = 0;
+    int rp = 1;
 };
 Legacy g_leg;
 
@@ -87,6 +88,7 @@ void cuda_application_init_data(int n_c, int nc3, int n, int n3, int *type, doub
     g_leg.n = n; g_leg.n_c = n_c; g_leg.n_tail = n_tail; g_leg.nlh = n_lh_n + n_lh_c;
     g_leg.flags.assign(nc_lh_flag, nc_lh_flag + n_c);
     g_leg.n_LH = n_LH;
+    g_leg.rp = 1;
 }
 
 void main_cuda(int n_c, int nc3, int step, int number_of_steps, double time_step, double del, int frequency_RP,
@@ -97,7 +99,7 @@ void main_cuda(int n_c, int nc3, int step, int number_of_steps, double time_step
                int n_tail3, int n_LH, int n_LH3, double a1, double a2, double s2dt, double *rr, double *p,
                double *Energy, double *h_r, double *h_a, double *h_b, double *h_c, double *h_r_t, double *h_r_lh,
                double *h_rad_all, int *nc_lh_flag) {
-    (void)nc3; (void)frequency_RP; (void)h; (void)g; (void)s; (void)debyell; (void)debye; (void)q_l;
+    (void)nc3; (void)h; (void)g; (void)s; (void)debyell; (void)debye; (void)q_l;
     (void)k_e; (void)k_ex; (void)k_h1; (void)sigma_DNA_DNA; (void)sigma_DNA_Core; (void)sigma_Core_Core;
     (void)sigma_Tail_Tail; (void)sigma_Tail_Linker; (void)sigma_Tail_Core; (void)Nq; (void)Nq3; (void)n_t;
     (void)n_lh_n; (void)n_lh_g; (void)n_lh_c; (void)n3; (void)n_tail3; (void)n_LH3;
@@ -110,6 +112,15 @@ void main_cuda(int n_c, int nc3, int step, int number_of_steps, double time_step
         rc = bd_set_integrator(ctx, a1, a2, s2dt, del, time_step);
         if (rc) { report("main_cuda", rc); return; }
         g_leg.a1 = a1; g_leg.a2 = a2; g_leg.s2dt = s2dt; g_leg.del = del; g_leg.dt = time_step;
+    }
+    // frequency_RP is ignored by the reference (func_cuda.cu:3504);
+    // BD_FREQUENCY_RP=1 in the environment honours it
+    const char *fr = std::getenv("BD_FREQUENCY_RP");
+    const int want_rp = (fr && std::strcmp(fr, "1") == 0 && frequency_RP > 1) ? frequency_RP : 1;
+    if (want_rp != g_leg.rp) {
+        rc = bd_set_frequency_rp(ctx, want_rp);
+        if (rc) { report("main_cuda", rc); return; }
+        g_leg.rp = want_rp;
     }
     rc = bd_step(ctx, rr, p, Energy);
     if (rc) { report("main_cuda", rc); return; }

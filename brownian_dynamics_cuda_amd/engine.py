@@ -82,6 +82,8 @@ API = {
     "bd_get_tensor": ([C.c_void_p, _dp], C.c_int),
     "bd_noise_displacement": ([C.c_void_p, _dp, _dp], C.c_int),
     "bd_set_chol_mode": ([C.c_void_p, C.c_int], C.c_int),
+    "bd_set_frequency_rp": ([C.c_void_p, C.c_int], C.c_int),
+    "bd_run_steps": ([C.c_void_p, C.c_int, _dp, _dp, _dp], C.c_int),
     "bd_set_timing": ([C.c_void_p, C.c_int], C.c_int),
     "bd_get_timing": ([C.c_void_p, C.POINTER(BdKernelTime), C.c_int, _ip], C.c_int),
     "bd_get_work_counts": ([C.c_void_p, C.POINTER(BdWorkCounts)], C.c_int),
@@ -283,6 +285,17 @@ class BDEngine:
                                        C.c_void_p(d_energy) if d_energy else None),
                "bd_step_device")
 
+    def run_steps(self, rr, p):
+        """k steps with host noise rr[k, 3n], p[k, n_D3]; returns the k energies."""
+        rr = np.ascontiguousarray(rr, np.float64)
+        p = np.ascontiguousarray(p, np.float64)
+        k = rr.shape[0]
+        if rr.shape != (k, 3 * self.n) or p.shape != (k, self.n_D3):
+            raise ValueError(f"rr needs shape ({k}, {3 * self.n}) and p ({k}, {self.n_D3})")
+        e = np.zeros(max(k, 1))
+        _check(self.lib.bd_run_steps(self.h, k, _pd(rr), _pd(p), _pd(e)), "bd_run_steps")
+        return e[:k]
+
     def synchronize(self):
         _check(self.lib.bd_synchronize(self.h), "bd_synchronize")
 
@@ -319,6 +332,10 @@ class BDEngine:
 
     def set_chol_mode(self, mode: int):
         _check(self.lib.bd_set_chol_mode(self.h, mode), "bd_set_chol_mode")
+
+    def set_frequency_rp(self, frequency_rp: int):
+        """Rebuild D and L every `frequency_rp` steps (1: every step)."""
+        _check(self.lib.bd_set_frequency_rp(self.h, int(frequency_rp)), "bd_set_frequency_rp")
 
     def set_timing(self, on: bool):
         _check(self.lib.bd_set_timing(self.h, 1 if on else 0), "bd_set_timing")

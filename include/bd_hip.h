@@ -124,6 +124,13 @@ int bd_step(bd_context *ctx, const double *rr, const double *p, double *energy);
  * double (may be NULL).  No host synchronisation. */
 int bd_step_device(bd_context *ctx, const double *d_rr, const double *d_p, double *d_energy);
 
+/* k consecutive steps with host noise rr[k][3n] and p[k][3(n+n_tail+n_LH)]
+ * (step-major, the draw order of main.cpp:479-486 for each step) and the k
+ * pass-1 energies back in energies[k] (may be NULL): one upload, k steps,
+ * one download.  Returns after the last step has finished.  The host loop of
+ * main.cpp:384-512 between two linker-histone moves or two samples. */
+int bd_run_steps(bd_context *ctx, int k, const double *rr, const double *p, double *energies);
+
 /* Forces at the current state (one force pass, func_cuda.cu:3556-3613).
  * terms: bit mask BD_TERM_*.  Outputs are host arrays (NULL to skip). */
 #define BD_TERM_MECH 1
@@ -144,6 +151,15 @@ int bd_get_tensor(bd_context *ctx, double *D);
 /* rd = s2dt * L * p for the current factor (rd_cal, func_cuda.cu:634-647). */
 int bd_noise_displacement(bd_context *ctx, const double *p, double *rd);
 int bd_set_chol_mode(bd_context *ctx, int chol_mode);
+
+/* frequency_RP (main.cpp:45, read from setup.txt): rebuild D and its factor
+ * only every frequency_rp-th step and reuse L for rd = s2dt L p in between,
+ * the gate that func_cuda.cu:3504 has commented out.  1 (the default, and the
+ * reference's behaviour) rebuilds every step.  Counting restarts, with a
+ * rebuild, after bd_set_state, bd_set_chol_mode, bd_get_tensor and any
+ * bd_set_lh that changes the hydrodynamic size; the drift is unaffected
+ * (it uses the self-mobilities). */
+int bd_set_frequency_rp(bd_context *ctx, int frequency_rp);
 
 /* Per-kernel timing with HIP events on the context's stream (0 = off). */
 int bd_set_timing(bd_context *ctx, int enable);

@@ -208,6 +208,14 @@ def _setup(lib, prefix):
         lib.ref_rand_normal.restype = C.c_double
         lib.ref_twist_angle_phi.argtypes = [C.c_int]
         lib.ref_twist_angle_phi.restype = C.c_double
+        lib.ref_build_lh.argtypes = [C.c_int, _ip, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp,
+                                     _dp, _dp, _dp, _dp, _ip, _dp, _dp]
+        lib.ref_write_xyz_append.argtypes = ([C.c_int] * 5 + [_ip] + [C.c_int] * 2 + [_ip]
+                                             + [_dp] * 8 + [C.c_char_p])
+        lib.ref_write_restart.argtypes = [C.c_int] * 3 + [_dp] * 7
+        lib.ref_read_restart.argtypes = [C.c_int] * 3 + [_dp] * 7
+        lib.ref_initial_er.argtypes = [C.c_char_p, _dp, C.c_int]
+        lib.ref_initial_er.restype = C.c_int
     return lib
 
 

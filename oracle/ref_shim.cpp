@@ -8,8 +8,11 @@
 // predictor-corrector step from them in the order of main_cuda
 // (func_cuda.cu:3434-3680), which has no CPU driver in the reference.
 // Functions that write side files (Diffusion_Tensor -> wrongD.txt,
-// Linker_Histone_force -> wf.txt) write them into the current directory.
+// Linker_Histone_force -> wf.txt) write them into the current directory, and
+// so do the reference's writers of readfile.cpp (out.xyz, restart_*.txt),
+// which are exported here to pin the host driver's output formats.
 #include "func.h"
+#include "readfile.h"
 
 #include <cstring>
 #include <vector>
@@ -253,6 +256,49 @@ int ref_step(const ref_system *S, ref_state *X, const double *rr, const double *
     if (nl) update_LH_beta(nl, S->n_lh_c, X->r_lh, X->beta_lh);
     if (energy) *energy = E0;
     return 0;
+}
+
+// build_LH, func.cpp:2029-2059
+void ref_build_lh(int n, int *type, int n_LH, const double *LH_n_pos, int n_lh_n, const double *LH_c_pos,
+                  int n_lh_c, const double *LH_radius, double *r, double *a, double *b, double *c,
+                  int *nc_lh_flag, double *r_lh, double *lh_rad) {
+    const int nlh = n_lh_n + n_lh_c;
+    build_LH(n, type, n_LH, vd(LH_n_pos, 3 * n_lh_n), vd(LH_c_pos, 3 * n_lh_c), vd(LH_radius, nlh), r, a, b, c,
+             n_lh_n, n_lh_c, nc_lh_flag, r_lh, lh_rad);
+}
+
+// write_xyz_append, readfile.cpp:346-405 (appends to `filename`)
+void ref_write_xyz_append(int n, int n_c, int n_tail, int n_lh_g, int nc_lh, int *nc_lh_flag, int n_LH, int Nq,
+                          int *type, double *r, double *a, double *b, double *c, const double *core_pos,
+                          double *r_t, const double *LH_g_pos, double *r_lh, const char *filename) {
+    write_xyz_append(n, 3 * n, n_c, n_tail, n_lh_g, nc_lh, nc_lh_flag, n_LH, Nq, 3 * Nq, type, r, a, b, c,
+                     vd(core_pos, 3 * Nq), r_t, vd(LH_g_pos, 3 * n_lh_g), r_lh, std::string(filename));
+}
+
+// write_restart, readfile.cpp:407-441 (restart_*.txt in the current directory)
+void ref_write_restart(int n3, int n_tail3, int n_LH3, double *r, double *a, double *b, double *c, double *Er,
+                       double *r_t, double *r_lh) {
+    write_restart(n3, n_tail3, n_LH3, r, a, b, c, Er, r_t, r_lh);
+}
+
+// Read_restart_ini/_tail/_LH, readfile.cpp:443-561 (current directory)
+void ref_read_restart(int n3, int n_tail3, int n_LH3, double *r, double *a, double *b, double *c, double *Er,
+                      double *r_t, double *r_lh) {
+    Read_restart_ini(n3, r, a, b, c, Er);
+    if (n_tail3) Read_restart_tail(n_tail3, r_t);
+    if (n_LH3) Read_restart_LH(n_LH3, r_lh);
+}
+
+// Read_initial_struc, readfile.cpp:40-134: the per-coordinate drag
+// coefficients Er of `filename` (3 per bead)
+int ref_initial_er(const char *filename, double *Er, int max3) {
+    int n, n_c, n3, nc3;
+    std::vector<double> r, a, b, c, rad, er;
+    std::vector<int> type;
+    Read_initial_struc(n, n_c, n3, nc3, r, a, b, c, rad, er, type, std::string(filename));
+    if ((int)er.size() > max3) return -1;
+    std::memcpy(Er, er.data(), sizeof(double) * er.size());
+    return (int)er.size();
 }
 
 }  // extern "C"
