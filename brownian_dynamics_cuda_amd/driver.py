@@ -259,10 +259,13 @@ class Setup:
                      unbind=v["unbind"], Pbind=v["Pbind"])
 
 
-def load_deck(deck: str, hi_all: int = 1, q_l: Optional[float] = None):
+def load_deck(deck: str, hi_all: int = 1, q_l: Optional[float] = None,
+              groups_per_core: int = 10):
     """Fiber, start state, Er and setup of a deck directory, as main.cpp:40-283
     assembles them (restart = 1 takes r, a, b, c, Er, r_t and r_lh from the
-    restart files of the same directory)."""
+    restart files of the same directory).  q_l (None: derived from Cs, as
+    main.cpp:78-84) and groups_per_core (the literal 10 of func.cpp:2368)
+    override the reference's values for decks written from other fibers."""
     su = Setup.read(os.path.join(deck, "setup.txt"))
     ph = S.Physics.derive(su.T, su.Cs, su.Mg, sigmas=su.sigmas, q_l=q_l)
     types, r, a, b, c, rad = S.read_data_mod(os.path.join(deck, "data_mod"))
@@ -276,7 +279,7 @@ def load_deck(deck: str, hi_all: int = 1, q_l: Optional[float] = None):
         n3 = 3 * len(types)
         r, a, b, c, er, _, _ = read_restart(deck, n3, 0, 0)
     fib, st = S.assemble(types, rad, r, a, b, c, core_pos, core_q, tl, lh, flags, ph,
-                         hi_all=hi_all)
+                         hi_all=hi_all, groups_per_core=groups_per_core)
     if su.restart == 1:
         _, _, _, _, _, r_t, r_lh = read_restart(deck, 3 * fib.n, 3 * fib.n_tail, 3 * fib.n_LH)
         st = State(r=st.r, a=st.a, b=st.b, c=st.c, r_t=r_t, r_lh=r_lh)
@@ -299,12 +302,14 @@ class Run:
                  steps: Optional[int] = None, hi_all: int = 1, chol_mode: Optional[int] = None,
                  use_frequency_rp: bool = False, fix_stale_frames: bool = False,
                  resume_rng: bool = False, resume: Optional[str] = None, batch: int = 100,
-                 write_checkpoint: bool = True, engine=None, rng=None, q_l: Optional[float] = None):
+                 write_checkpoint: bool = True, engine=None, rng=None, q_l: Optional[float] = None,
+                 groups_per_core: int = 10):
         from . import engine as E
         self.deck = deck
         self.out = out or deck
         os.makedirs(self.out, exist_ok=True)
-        self.fib, st, self.Er, self.setup = load_deck(deck, hi_all=hi_all, q_l=q_l)
+        self.fib, st, self.Er, self.setup = load_deck(deck, hi_all=hi_all, q_l=q_l,
+                                                      groups_per_core=groups_per_core)
         self.nlh = self.fib.n_lh_n + self.fib.n_lh_c
         self.step0 = 0
         self.seed = int(time.time()) & 0xFFFFFFFF if seed is None else int(seed)

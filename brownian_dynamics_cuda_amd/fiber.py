@@ -43,6 +43,36 @@ def _entry_point(r, a, b, c):
     return r - S.RO * bm + S.D1 * c
 
 
+def chain_geometry(types, r, a, b, c):
+    """Conventions of a bead chain as the reference's frame update reads them
+    (update_Euler_Angle, func.cpp:448-505; first_coord/second_coord,
+    func.cpp:687-723): frame orthonormality and handedness, the bond vector
+    from each linker bead to the next bead (a core's entry point), the
+    distance from each core's exit point to the bead after it, and the
+    alignment of each linker bead's a axis with its bond."""
+    types = np.asarray(types)
+    r, a, b, c = (np.asarray(x, np.float64).reshape(-1, 3) for x in (r, a, b, c))
+    dots = np.concatenate([np.einsum("ij,ij->i", a, b), np.einsum("ij,ij->i", a, c),
+                           np.einsum("ij,ij->i", b, c)])
+    norms = np.concatenate([np.linalg.norm(x, axis=1) for x in (a, b, c)])
+    hand = np.einsum("ij,ij->i", np.cross(a, b), c)
+    bonds, exits, align = [], [], []
+    for i in range(len(types) - 1):
+        if types[i] == 0:
+            if types[i + 1] == 0:
+                f = r[i + 1] - r[i]
+            else:
+                f = _entry_point(r[i + 1], a[i + 1], b[i + 1], c[i + 1]) - r[i]
+            bonds.append(np.linalg.norm(f))
+            align.append(np.dot(f, a[i]) / np.linalg.norm(f))
+        else:
+            exits.append(np.linalg.norm(r[i + 1] - _exit_point(r[i], b[i], c[i])))
+    cores = np.nonzero(types == 1)[0]
+    return dict(max_dot=float(np.max(np.abs(dots))), max_norm_err=float(np.max(np.abs(norms - 1))),
+                min_handedness=float(np.min(hand)), bonds=np.array(bonds), exits=np.array(exits),
+                align=np.array(align), linkers_per_core=np.diff(cores))
+
+
 def zigzag_geometry(n_c, linker_beads=6, radius=13.2, twist=0.85 * math.pi, rise=2.1,
                     rng=None):
     """Core centres on a two-start zigzag; straight linkers between the exit

@@ -367,6 +367,81 @@ def read_lh_in(path, n_c):
     return np.array(vals[:n_c], np.int32)
 
 
+# ---------------------------------------------------------------------------
+# writers of the reference input formats (inverse of the readers above)
+# ---------------------------------------------------------------------------
+
+_E_UNIT = 4.142e-3 / 0.5962  # energy scaling of the tail and LH force constants
+
+
+def _g(x):
+    return "%.17g" % float(x)
+
+
+def write_deck(directory, fib: "Fiber", st: "State", number_of_steps=1000,
+               frequency_of_sampling=100, frequency_RP=20, restart=0, unbind=0.0, Pbind=0.0):
+    """Writes a fiber and its state as an input deck the reference driver and
+    load_reference_deck read (setup.txt, data_mod, core_data.reg.150mM,
+    tail_data.mod.200mM, LH_N0G6C22.in, LH.in), undoing the readers' unit
+    scalings.  Numbers carry 17 significant digits.  The reference derives
+    q_l from Cs (main.cpp:78-84) and uses 10 tail groups per core
+    (func.cpp:2368); a fiber built with other values needs them passed
+    again when the deck is read."""
+    os.makedirs(directory, exist_ok=True)
+    j = lambda name: os.path.join(directory, name)
+    ph = fib.phys
+    rows = [("Temperature", ph.T), ("number_of_steps", number_of_steps),
+            ("frequency_of_sampling", frequency_of_sampling), ("frequency_RP", frequency_RP),
+            ("evd_DNA_DNA", ph.sig_dd), ("evd_DNA_Core", ph.sig_dc), ("evd_Core_Core", ph.sig_cc),
+            ("evd_Tail_Tail", ph.sig_tt), ("evd_Tail_DNA", ph.sig_tl), ("evd_Tail_Core", ph.sig_tc),
+            ("Mg", ph.Mg), ("Cs", ph.Cs), ("restart", restart), ("unbind", unbind),
+            ("Pbind", Pbind)]
+    with open(j("setup.txt"), "w") as f:
+        f.write("$ configuration file for the Brownian dynamics simulation\n")
+        for k, v in rows:
+            f.write(f"{k} {v if isinstance(v, int) else _g(v)}\n")
+    r, a, b, c = (np.asarray(x, np.float64).reshape(-1, 3) for x in (st.r, st.a, st.b, st.c))
+    with open(j("data_mod"), "w") as f:
+        for i in range(fib.n):
+            f.write(f"{_g(r[i, 0])} {_g(r[i, 1])} {_g(r[i, 2])} {int(fib.types[i])}\n")
+            for v in (a[i], b[i], c[i]):
+                f.write(f"{_g(v[0])} {_g(v[1])} {_g(v[2])}\n")
+    with open(j("core_data.reg.150mM"), "w") as f:
+        for p in fib.core_pos.reshape(-1, 3):
+            f.write(" ".join(_g(10 * x) for x in p) + "\n")
+        for q in fib.core_q:
+            f.write(_g(q) + "\n")
+    nt = fib.n_t
+    with open(j("tail_data.mod.200mM"), "w") as f:
+        for k in range(nt):
+            p = fib.tail_pos.reshape(-1, 3)[k]
+            vals = [fib.t_q[k] / 1.12, fib.t_rad[k] * 10, 0.0, 10 * p[0], 10 * p[1], 10 * p[2],
+                    fib.lo_t[k] * 10, fib.h_t[k] / (100 * _E_UNIT), fib.beta_o_t[k] * 180 / PI,
+                    fib.g_t[k] / _E_UNIT]
+            f.write(f"T{k + 1} {int(fib.t_grp[k])} {int(fib.tail_fix_tpl[k])} "
+                    + " ".join(_g(v) for v in vals) + "\n")
+    nn, ng, ncc = fib.n_lh_n, fib.n_lh_g, fib.n_lh_c
+    with open(j("LH_N0G6C22.in"), "w") as f:
+        f.write("# bead position (x,y,z) charge VdW_hh VdW_hc VdW_hl VdW_ht kstr kben ktor Seq "
+                "Conn rstreq betaeq\n")
+        pos = {"N": fib.LH_n_pos.reshape(-1, 3), "G": fib.LH_g_pos.reshape(-1, 3),
+               "C": fib.LH_c_pos.reshape(-1, 3)}
+        k = 0
+        for sect, cnt in (("N-term", nn), ("G-head", ng), ("C-term", ncc)):
+            f.write(f"# {sect}\n")
+            for m in range(cnt):
+                p = pos[sect[0]][m]
+                vals = [p[0], p[1], p[2], fib.LH_q[k] / 0.66, fib.LH_vdw_hh[k], fib.LH_vdw_hc[k],
+                        fib.LH_vdw_hl[k], fib.LH_vdw_ht[k], fib.LH_kstr[k] / _E_UNIT,
+                        fib.LH_kben[k] / _E_UNIT, 0.0]
+                f.write(" ".join(_g(v) for v in vals)
+                        + f" {k + 1} {int(fib.LH_conn[k])} {_g(fib.LH_streq[k])} "
+                        + _g(fib.LH_betaeq[k] * 180 / PI) + "\n")
+                k += 1
+    with open(j("LH.in"), "w") as f:
+        f.write("".join(f"{int(x)}\n" for x in fib.nc_lh_flag))
+
+
 def load_reference_deck(directory, hi_all=0, q_l=None):
     """Assemble a Fiber and State from a directory in the reference formats,
     as main.cpp:40-283 does (restart=0, every core tailed)."""

@@ -412,3 +412,30 @@ def test_frequency_rp_is_opt_in(tmp_path):
     e2 = RecordingEngine(fib)
     D.Run(deck, out=str(tmp_path / "o2"), seed=1, engine=e2, use_frequency_rp=True)
     assert e2.rp == 20  # setup.txt of the deck
+
+
+def test_deck_writer_round_trip(tmp_path):
+    """A generated fiber written in the reference formats reads back as the
+    same fiber and state (17 significant digits, the readers' unit scalings
+    undone)."""
+    from brownian_dynamics_cuda_amd import fiber as Fb
+    fib, st = Fb.make_fiber(4, seed=5, with_lh=True, hi_all=1)
+    d = str(tmp_path / "gen")
+    S.write_deck(d, fib, st, number_of_steps=300, frequency_of_sampling=25)
+    fib2, st2, er, su = D.load_deck(d, hi_all=1, q_l=fib.phys.q_l,
+                                    groups_per_core=fib.groups_per_core)
+    assert (su.number_of_steps, su.frequency_of_sampling, su.restart) == (300, 25, 0)
+    for k in ("r", "a", "b", "c", "r_t", "r_lh"):
+        np.testing.assert_allclose(getattr(st2, k), getattr(st, k), rtol=1e-15, atol=1e-15,
+                                   err_msg=k)
+    for f in dataclasses.fields(fib):
+        x, y = getattr(fib2, f.name), getattr(fib, f.name)
+        if f.name == "phys":
+            for g in dataclasses.fields(y):
+                assert getattr(x, g.name) == pytest.approx(getattr(y, g.name), rel=1e-14), g.name
+        elif isinstance(y, np.ndarray):
+            assert x.shape == y.shape, f.name
+            np.testing.assert_allclose(x, y, rtol=1e-14, atol=1e-15, err_msg=f.name)
+        else:
+            assert x == y, f.name
+    np.testing.assert_array_equal(er, D.initial_er(fib.types))

@@ -12,6 +12,7 @@ import oracle_py as O
 from brownian_dynamics_cuda_amd import engine as E
 from brownian_dynamics_cuda_amd import fiber as Fb
 from brownian_dynamics_cuda_amd import system as S
+from decks import GOLDEN
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -123,6 +124,30 @@ def test_benchmark_fiber_sizes(idx, n, nt, nl):
     assert d.min() > 12.0
     fib2, st2 = Fb.make_config(idx)
     assert np.array_equal(st.r, st2.r) and np.array_equal(fib.core_q, fib2.core_q)
+
+
+@pytest.mark.parametrize("idx", [1, 2, 4, 5])
+def test_generated_fibers_follow_data_mod_conventions(idx):
+    """The seeded fibers keep the chain conventions of the reference deck's
+    data_mod (tests/golden/testdata): core first, a fixed number of linker
+    beads after each core, orthonormal right-handed frames, linker a axes
+    along their bonds, and bond and exit distances in data_mod's range."""
+    ref = Fb.chain_geometry(*S.read_data_mod(os.path.join(GOLDEN, "testdata", "data_mod"))[:5])
+    fib, st = Fb.make_config(idx)
+    g = Fb.chain_geometry(fib.types, st.r, st.a, st.b, st.c)
+    assert fib.types[0] == 1
+    assert len(set(g["linkers_per_core"])) == 1 == len(set(ref["linkers_per_core"]))
+    for gg in (ref, g):
+        assert gg["max_dot"] < 1e-12 and gg["max_norm_err"] < 1e-12
+        assert gg["min_handedness"] > 1 - 1e-12
+    lo = min(ref["bonds"].min(), ref["exits"].min())
+    hi = max(ref["bonds"].max(), ref["exits"].max())
+    assert lo <= g["bonds"].min() and g["bonds"].max() <= hi
+    assert lo <= g["exits"].min() and g["exits"].max() <= hi
+    assert g["align"].min() >= ref["align"].min()
+    # mean bond within 10% of the reference deck's and of lo = 3 nm (constants.h)
+    assert abs(g["bonds"].mean() - ref["bonds"].mean()) < 0.1 * ref["bonds"].mean()
+    assert abs(g["bonds"].mean() - S.LO) < 0.1 * S.LO
 
 
 def test_reference_format_readers(tmp_path):
