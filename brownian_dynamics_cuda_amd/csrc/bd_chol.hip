@@ -8,11 +8,11 @@
 // Matrix: row-major, lower triangle, padded to a multiple of 64 with an
 // identity block (so the padded factor is [[L,0],[0,I]] and no kernel needs
 // a bounds test).  Right-looking blocked algorithm with 64x64 tiles, one
-// launch per block column kb (k_step): every workgroup solves the two
-// panel tiles it needs (substitution, a wavefront each, FP64), updates its
+// launch per block column kb (k_step): every workgroup forms the two panel
+// tiles it needs as A_ib,kb inv(L_kb,kb)^T (FP64 matrix cores), updates its
 // trailing tile A_ib,jb -= L_ib,kb L_jb,kb^T (FP64 or FP32 matrix cores),
-// and the workgroup owning tile (kb+1,kb+1) then factors it (one
-// wavefront, FP64).
+// and the workgroup owning tile (kb+1,kb+1) then factors it and inverts
+// the factor (two wavefronts, FP64, the inverse one panel behind).
 #include "bd_dev.h"
 #include "bd_kernels.h"
 
@@ -21,6 +21,7 @@ namespace bd {
 constexpr int TB = 64;        // tile edge
 constexpr int LDP = TB + 1;   // LDS row pitch (doubles)
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef double dbl2 __attribute__((ext_vector_type(2)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
 static inline int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
@@ -82,17 +83,32 @@ __global__ void __launch_bounds__(256) k_rpy(Stat s, Geo g, const double *rad_al
 // ---------------------------------------------------------------------------
 // tile helpers
 // ---------------------------------------------------------------------------
+// Tile copies between global memory and LDS (pitch LDP) by the 256 threads
+// of a workgroup, 16-byte accesses, fully unrolled: all eight global loads
+// of a thread are in flight before the first LDS store waits on them.
+constexpr int NTHR = 256;
 __device__ __forceinline__ void load_tile(const double *A, int ld, int r0, int c0, double *S) {
-    for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
-        int r = e / TB, c = e % TB;
-        S[r * LDP + c] = A[(size_t)(r0 + r) * ld + c0 + c];
+    dbl2 v[TB * TB / 2 / NTHR];
+#pragma unroll
+    for (int i = 0; i < TB * TB / 2 / NTHR; i++) {
+        const int e = threadIdx.x + i * NTHR, r = e / (TB / 2), c = 2 * (e % (TB / 2));
+        v[i] = *reinterpret_cast<const dbl2 *>(A + (size_t)(r0 + r) * ld + c0 + c);
+    }
+#pragma unroll
+    for (int i = 0; i < TB * TB / 2 / NTHR; i++) {
+        const int e = threadIdx.x + i * NTHR, r = e / (TB / 2), c = 2 * (e % (TB / 2));
+        S[r * LDP + c] = v[i].x;
+        S[r * LDP + c + 1] = v[i].y;
     }
 }
 __device__ __forceinline__ void store_tile(double *A, int ld, int r0, int c0, const double *S, bool lower_only) {
-    for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
-        int r = e / TB, c = e % TB;
-        if (lower_only && c > r) continue;
-        A[(size_t)(r0 + r) * ld + c0 + c] = S[r * LDP + c];
+#pragma unroll
+    for (int i = 0; i < TB * TB / 2 / NTHR; i++) {
+        const int e = threadIdx.x + i * NTHR, r = e / (TB / 2), c = 2 * (e % (TB / 2));
+        dbl2 v;
+        v.x = (lower_only && c > r) ? 0.0 : S[r * LDP + c];
+        v.y = (lower_only && c + 1 > r) ? 0.0 : S[r * LDP + c + 1];
+        if (!lower_only || c <= r) *reinterpret_cast<dbl2 *>(A + (size_t)(r0 + r) * ld + c0 + c) = v;
     }
 }
 
@@ -167,10 +183,12 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 // 16-byte LDS loads and applies 8 FMAs.
 // ---------------------------------------------------------------------------
 // Rank-8 update of the columns c >= C0 of a row held in a[]: a[c] -=
-// sum_q x[q] * lp[c*8 + q].  The 8 panel entries of column c are 4
-// broadcast 16-byte LDS loads, issued two columns ahead of their FMAs (a
-// scheduling barrier per column keeps that distance; left to itself the
-// scheduler reuses one register quad and exposes every LDS latency).
+// sum_q x[q] * lp[c*8 + q].  Columns go two at a time, their two 8-FMA
+// chains interleaved so that one chain's FMA latency is covered by the
+// other's; the 8 panel entries of a column are 4 broadcast 16-byte LDS
+// loads, issued one column pair ahead of their FMAs (a scheduling barrier
+// per pair keeps that distance; left to itself the scheduler reuses one
+// register quad and exposes every LDS latency).  C0 is even.
 template <int C0>
 __device__ __forceinline__ void rank8_update(double (&a)[TB], const double (&x)[8], const double *lp) {
     const double2 *l2 = reinterpret_cast<const double2 *>(lp);
@@ -180,23 +198,29 @@ __device__ __forceinline__ void rank8_update(double (&a)[TB], const double (&x)[
 #pragma unroll
     for (int k = 0; k < 4; k++) p1[k] = (C0 + 1 < TB) ? l2[(C0 + 1) * 4 + k] : double2{0.0, 0.0};
 #pragma unroll
-    for (int c = C0; c < TB; c++) {
-        double2 cur[4];
+    for (int c = C0; c < TB; c += 2) {
+        double2 u[4], v[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            cur[k] = p0[k];
-            p0[k] = p1[k];
+            u[k] = p0[k];
+            v[k] = p1[k];
         }
-        if (c + 2 < TB)
+        if (c + 3 < TB)
 #pragma unroll
-            for (int k = 0; k < 4; k++) p1[k] = l2[(c + 2) * 4 + k];
-        double acc = a[c];
+            for (int k = 0; k < 4; k++) {
+                p0[k] = l2[(c + 2) * 4 + k];
+                p1[k] = l2[(c + 3) * 4 + k];
+            }
+        double s0 = a[c], s1 = a[c + 1];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            acc -= x[2 * k] * cur[k].x;
-            acc -= x[2 * k + 1] * cur[k].y;
+            s0 -= x[2 * k] * u[k].x;
+            s1 -= x[2 * k] * v[k].x;
+            s0 -= x[2 * k + 1] * u[k].y;
+            s1 -= x[2 * k + 1] * v[k].y;
         }
-        a[c] = acc;
+        a[c] = s0;
+        a[c + 1] = s1;
         __builtin_amdgcn_sched_barrier(0);
     }
 }
@@ -279,86 +303,92 @@ __device__ __forceinline__ void store_row(double *A, int ld, int r0, int c0, con
     }
 }
 
-// Wavefront 0 of the calling workgroup factors the tile T (LDS, pitch LDP)
-// at (d0, d0): L goes to A (lower, zeros above), the factor in panel order
-// to Lp (global copy of LP for the panel solves) and the pivot inverses to
-// invd.  LP (64*64) is LDS scratch.
-__device__ __forceinline__ void diag_factor(const double *T, double *LP, double *A, int ld, int d0, double *Lp,
-                                            double *invd, int *neg) {
-    const int lane = threadIdx.x & 63;
-    double a[TB];
+// Called by all four wavefronts of the workgroup.  Wavefront 0 factors the
+// tile T (LDS, pitch LDP) at (d0, d0) one panel of 8 columns at a time,
+// publishing each panel to LP (LDS scratch, 64*64) and its pivot inverses
+// to ivs (LDS, 64).  Wavefront 1 follows one panel behind and solves
+// x L^T = e_lane, which leaves column `lane` of inv(L) in its row: the
+// inverse costs one panel of latency beyond the factor.  L goes to A (lower,
+// zeros above) and inv(L), row-major 64x64, to Linv.
+__device__ __forceinline__ void diag_factor_inv(const double *T, double *LP, double *ivs, double *A, int ld,
+                                                int d0, double *Linv, int *neg) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double r[TB];
 #pragma unroll
-    for (int c = 0; c < TB; c++) a[c] = T[lane * LDP + c];
+    for (int c = 0; c < TB; c++) r[c] = (w == 0) ? T[lane * LDP + c] : (c == lane ? 1.0 : 0.0);
     int nneg = 0;
     double myinv = 0.0;
-    factor_panel8<0>(a, LP, myinv, nneg);
-    factor_panel8<8>(a, LP, myinv, nneg);
-    factor_panel8<16>(a, LP, myinv, nneg);
-    factor_panel8<24>(a, LP, myinv, nneg);
-    factor_panel8<32>(a, LP, myinv, nneg);
-    factor_panel8<40>(a, LP, myinv, nneg);
-    factor_panel8<48>(a, LP, myinv, nneg);
-    factor_panel8<56>(a, LP, myinv, nneg);
-    store_row(A, ld, d0, d0, a, true);
-#pragma unroll 8
-    for (int e = 0; e < TB * TB / 2; e += TB)
-        reinterpret_cast<double2 *>(Lp)[e + lane] = reinterpret_cast<const double2 *>(LP)[e + lane];
-    invd[d0 + lane] = myinv;
-    if (nneg && lane == 0) atomicAdd(neg, nneg);
+#define BD_DIAG_STEP(P)                                                        \
+    if (w == 0) {                                                              \
+        factor_panel8<P>(r, LP, myinv, nneg);                                  \
+        if (lane >= (P) && lane < (P) + 8) ivs[lane] = myinv;                  \
+    } else if (w == 1 && (P) > 0) {                                            \
+        solve_panel8<((P) > 0 ? (P) - 8 : 0)>(r, LP, ivs);                     \
+    }                                                                          \
+    __syncthreads();
+    BD_DIAG_STEP(0)
+    BD_DIAG_STEP(8)
+    BD_DIAG_STEP(16)
+    BD_DIAG_STEP(24)
+    BD_DIAG_STEP(32)
+    BD_DIAG_STEP(40)
+    BD_DIAG_STEP(48)
+    BD_DIAG_STEP(56)
+#undef BD_DIAG_STEP
+    if (w == 1) {
+        solve_panel8<56>(r, LP, ivs);
+#pragma unroll
+        for (int c = 0; c < TB; c++) Linv[c * TB + lane] = r[c];
+    } else if (w == 0) {
+        store_row(A, ld, d0, d0, r, true);
+        if (nneg && lane == 0) atomicAdd(neg, nneg);
+    }
 }
 
 // first diagonal tile
-__global__ void __launch_bounds__(256) k_potrf0(double *A, int ld, double *Lp, double *invd, int *neg) {
-    __shared__ double T[TB * LDP], LP[TB * TB];
+__global__ void __launch_bounds__(256) k_potrf0(double *A, int ld, double *Linv, int *neg) {
+    __shared__ double T[TB * LDP], LP[TB * TB], ivs[TB];
     load_tile(A, ld, 0, 0, T);
     __syncthreads();
-    if ((threadIdx.x >> 6) == 0) diag_factor(T, LP, A, ld, 0, Lp, invd, neg);
+    diag_factor_inv(T, LP, ivs, A, ld, 0, Linv, neg);
 }
 
 // One block column kb of the factorisation in one launch, one workgroup per
 // trailing lower tile (ib, jb), kb < jb <= ib:
-//   1. the panel tiles L_ib,kb and L_jb,kb are solved from A against the
-//      diagonal factor of step kb (wavefronts 0 and 1, row per lane,
-//      X L_kb,kb^T = A); the workgroup of the diagonal tile (ib, ib)
-//      stores L_ib,kb, so every panel tile is written exactly once, into the
-//      upper block (kb, ib) (k_panel_copy moves them after the last step);
+//   1. the panel tiles L_ib,kb = A_ib,kb inv(L_kb,kb)^T and L_jb,kb on the
+//      FP64 matrix cores, from the inverse the previous launch left in
+//      Linv; the workgroup of the diagonal tile (ib, ib) stores L_ib,kb, so
+//      every panel tile is written exactly once, into the upper block
+//      (kb, ib), because other workgroups of this launch still read
+//      A_ib,kb (k_panel_copy moves them after the last step);
 //   2. A_ib,jb -= L_ib,kb L_jb,kb^T on the matrix cores (FP64, or FP32 with
 //      FP32 accumulation when MODE != 0);
-//   3. the workgroup of tile (kb+1, kb+1) factors it (wavefront 0), which
-//      the next launch needs.
-// The panel solves are repeated by the workgroups of a block row, in
+//   3. the workgroup of tile (kb+1, kb+1) factors and inverts it, which the
+//      next launch needs.
+// The panel products are repeated by the workgroups of a block row, in
 // exchange for one launch (and one kernel boundary) per block column.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, double *Lp, double *invd, int *neg) {
-    __shared__ double LPs[TB * TB], iv[TB], As[TB * LDP], Bs[TB * LDP];
+__global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, double *Linv, int *neg) {
+    __shared__ double W[TB * LDP], As[TB * LDP], Bs[TB * LDP];
     int ii, jj;
     tri_index(blockIdx.x, ii, jj);
     const int ib = kb + 1 + ii, jb = kb + 1 + jj;
-    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    for (int e = tid; e < TB * TB / 2; e += blockDim.x)
-        reinterpret_cast<double2 *>(LPs)[e] = reinterpret_cast<const double2 *>(Lp + (size_t)kb * TB * TB)[e];
-    if (tid < TB) iv[tid] = invd[kb * TB + tid];
+    load_tile(Linv + (size_t)kb * TB * TB, TB, 0, 0, W);
+    load_tile(A, ld, ib * TB, kb * TB, As);
+    if (ib != jb) load_tile(A, ld, jb * TB, kb * TB, Bs);
     __syncthreads();
-    if (w == 0 || (w == 1 && ib != jb)) {
-        const int rb = (w == 0) ? ib : jb;
-        double b[TB];
-        load_row(A, ld, rb * TB, kb * TB, b);
-        solve_panel8<0>(b, LPs, iv);
-        solve_panel8<8>(b, LPs, iv);
-        solve_panel8<16>(b, LPs, iv);
-        solve_panel8<24>(b, LPs, iv);
-        solve_panel8<32>(b, LPs, iv);
-        solve_panel8<40>(b, LPs, iv);
-        solve_panel8<48>(b, LPs, iv);
-        solve_panel8<56>(b, LPs, iv);
-        double *S = (w == 0) ? As : Bs;
-#pragma unroll
-        for (int c = 0; c < TB; c++) S[lane * LDP + c] = b[c];
-        // the other workgroups of this launch still read A_ib,kb: the solved
-        // tile goes to the unused upper block (kb, ib), moved by k_panel_copy
-        if (w == 0 && ib == jb) store_row(A, ld, kb * TB, ib * TB, b, false);
+    {
+        dbl4 pi[2][2], pj[2][2];
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) pi[a][b] = pj[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+        mma_f64(As, W, pi);
+        if (ib != jb) mma_f64(Bs, W, pj);
+        __syncthreads();
+        acc_to_lds(As, pi, 1.0, true);
+        if (ib != jb) acc_to_lds(Bs, pj, 1.0, true);
     }
     __syncthreads();
+    if (ib == jb) store_tile(A, ld, kb * TB, ib * TB, As, false);
     const double *Bp = (ib != jb) ? Bs : As;
     if (MODE == 0) {
         dbl4 acc[2][2];
@@ -366,25 +396,24 @@ __global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, double 
             for (int b = 0; b < 2; b++) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
         mma_f64(As, Bp, acc);
         __syncthreads();
-        load_tile(A, ld, ib * TB, jb * TB, As);
+        load_tile(A, ld, ib * TB, jb * TB, W);
         __syncthreads();
-        acc_to_lds(As, acc, -1.0, false);
+        acc_to_lds(W, acc, -1.0, false);
     } else {
         flt4 acc[2][2];
         for (int a = 0; a < 2; a++)
             for (int b = 0; b < 2; b++) acc[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
         mma_f32(As, Bp, acc);
         __syncthreads();
-        load_tile(A, ld, ib * TB, jb * TB, As);
+        load_tile(A, ld, ib * TB, jb * TB, W);
         __syncthreads();
-        acc_to_lds(As, acc, -1.0, false);
+        acc_to_lds(W, acc, -1.0, false);
     }
     __syncthreads();
-    if (ii == 0 && jj == 0) {
-        if (w == 0) diag_factor(As, Bs, A, ld, ib * TB, Lp + (size_t)ib * TB * TB, invd, neg);
-    } else {
-        store_tile(A, ld, ib * TB, jb * TB, As, ib == jb);
-    }
+    if (ii == 0 && jj == 0)
+        diag_factor_inv(W, As, Bs, A, ld, ib * TB, Linv + (size_t)ib * TB * TB, neg);
+    else
+        store_tile(A, ld, ib * TB, jb * TB, W, ib == jb);
 }
 
 // rd = s2dt * L * p (rd_cal, func_cuda.cu:634-647): one wavefront per row,
@@ -418,10 +447,16 @@ __global__ void __launch_bounds__(256) k_panel_copy(double *A, int ld) {
     int ii, jj;
     tri_index(blockIdx.x, ii, jj);
     const int ib = ii + 1, kb = jj;
-    for (int e = threadIdx.x; e < TB * TB / 2; e += blockDim.x) {
-        const int r = e / (TB / 2), c = 2 * (e % (TB / 2));
-        *reinterpret_cast<double2 *>(A + (size_t)(ib * TB + r) * ld + kb * TB + c) =
-            *reinterpret_cast<const double2 *>(A + (size_t)(kb * TB + r) * ld + ib * TB + c);
+    dbl2 v[TB * TB / 2 / NTHR];
+#pragma unroll
+    for (int i = 0; i < TB * TB / 2 / NTHR; i++) {
+        const int e = threadIdx.x + i * NTHR, r = e / (TB / 2), c = 2 * (e % (TB / 2));
+        v[i] = *reinterpret_cast<const dbl2 *>(A + (size_t)(kb * TB + r) * ld + ib * TB + c);
+    }
+#pragma unroll
+    for (int i = 0; i < TB * TB / 2 / NTHR; i++) {
+        const int e = threadIdx.x + i * NTHR, r = e / (TB / 2), c = 2 * (e % (TB / 2));
+        *reinterpret_cast<dbl2 *>(A + (size_t)(ib * TB + r) * ld + kb * TB + c) = v[i];
     }
 }
 
@@ -430,15 +465,15 @@ void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_a
     auto M_ = [&](int id, int ph) { if (mark) mark(mark_arg, id, ph); };
     launch_fill_int(c.neg, 1, 0, q);
     M_(K_POTRF, 0);
-    k_potrf0<<<1, 256, 0, q>>>(c.A, c.ld, c.Lp, c.invd, c.neg);
+    k_potrf0<<<1, 256, 0, q>>>(c.A, c.ld, c.Linv, c.neg);
     M_(K_POTRF, 1);
     for (int kb = 0; kb + 1 < nt; kb++) {
         const int M = nt - kb - 1;
         M_(K_CHOL_STEP, 0);
         if (c.mode == 0)
-            k_step<0><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Lp, c.invd, c.neg);
+            k_step<0><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
         else
-            k_step<1><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Lp, c.invd, c.neg);
+            k_step<1><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
         M_(K_CHOL_STEP, 1);
     }
     if (nt > 1) {

@@ -455,8 +455,7 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     c->chol.ld = ((3 * nhi_cap + 63) / 64) * 64;
     c->chol.mode = o.chol_mode;
     r |= c->alloc(&c->chol.A, (size_t)c->chol.ld * c->chol.ld);
-    r |= c->alloc(&c->chol.Lp, (size_t)(c->chol.ld / 64) * 64 * 64);
-    r |= c->alloc(&c->chol.invd, c->chol.ld);
+    r |= c->alloc(&c->chol.Linv, (size_t)(c->chol.ld / 64) * 64 * 64);
     r |= c->alloc(&c->chol.neg, 1);
     r |= c->alloc(&c->rd, 3 * nD_cap);
     r |= c->alloc(&c->p, 3 * nD_cap);

@@ -43,8 +43,7 @@ void launch_commit(const Stat &s, const Geo &from, const Geo &to, hipStream_t q)
 struct CholWork {
     double *A;       // mpad x ld, row-major; lower triangle holds D then L
     int m, mpad, ld;
-    double *Lp;      // (ld/64) tiles of 64x64: diagonal factors in panel order
-    double *invd;    // ld: pivot inverses 1 / L_jj
+    double *Linv;    // (ld/64) tiles of 64x64: inverses of the diagonal factors, row-major
     int *neg;        // negative-pivot counter
     int mode;        // BD_CHOL_*
 };
