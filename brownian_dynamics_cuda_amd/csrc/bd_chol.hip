@@ -174,44 +174,53 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal tile factor (one wavefront) and the panel solve, one row per
-// lane in registers, in panels of 8 columns.
+// Diagonal tile: factor and inverse by the four wavefronts of a workgroup.
 //
-// LP holds the factor of the diagonal tile in panel order: LP[p][c][q] =
-// L[c][8p + q] (8 panels x 64 rows x 8 doubles, 32 KB), so the rank-8
-// update of a row reads the 8 panel entries of column c as 4 broadcast
-// 16-byte LDS loads and applies 8 FMAs.
+// Wavefront w holds columns 16w..16w+15 of every row (lane = row) in g[].
+// The 8 panels of 8 columns are factored in order by the wavefront that
+// owns them (panels 2w and 2w+1): the CPU algorithm of func.cpp:806-836 in
+// right-looking order -- pivot f (|f| and a count if f < 0), column scaled
+// by 1/sqrt(f) -- with the updates inside a panel taken from the owning
+// lanes by readlane.  A finished panel goes to LP in panel order,
+// LP[p][row][q] = L[row][8p + q], and while its owner goes on, the
+// wavefronts to its right apply it to their columns (rank-8 updates that
+// read the 8 entries of a column as broadcast 16-byte LDS loads).  Only
+// the owner's pivots, its catch-up on the previous group's last panel and
+// its in-group update are on the critical path.  A wavefront with no
+// columns left inverts its 16x16 diagonal block of L by substitution, and
+// the off-diagonal 16x16 blocks of X = inv(L) follow on the matrix cores,
+//   X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj.
 // ---------------------------------------------------------------------------
-// Rank-8 update of the columns c >= C0 of a row held in a[]: a[c] -=
-// sum_q x[q] * lp[c*8 + q].  Columns go two at a time, their two 8-FMA
-// chains interleaved so that one chain's FMA latency is covered by the
-// other's; the 8 panel entries of a column are 4 broadcast 16-byte LDS
-// loads, issued one column pair ahead of their FMAs (a scheduling barrier
-// per pair keeps that distance; left to itself the scheduler reuses one
-// register quad and exposes every LDS latency).  C0 is even.
-template <int C0>
-__device__ __forceinline__ void rank8_update(double (&a)[TB], const double (&x)[8], const double *lp) {
-    const double2 *l2 = reinterpret_cast<const double2 *>(lp);
+
+// g[C0 .. C0+NC) -= sum_q x[q] * lp[col*8 + q] for the columns col =
+// colbase + C0 + c.  Two columns at a time, their FMA chains interleaved;
+// the entries of the next pair are loaded one pair ahead (a scheduling
+// barrier per pair keeps that distance).
+template <int C0, int NC>
+__device__ __forceinline__ void panel_update16(double (&g)[16], const double (&x)[8], const double *lp,
+                                               int colbase) {
+    const double2 *l2 = reinterpret_cast<const double2 *>(lp) + colbase * 4;
     double2 p0[4], p1[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) p0[k] = (C0 < TB) ? l2[C0 * 4 + k] : double2{0.0, 0.0};
+    for (int k = 0; k < 4; k++) {
+        p0[k] = l2[C0 * 4 + k];
+        p1[k] = l2[(C0 + 1) * 4 + k];
+    }
 #pragma unroll
-    for (int k = 0; k < 4; k++) p1[k] = (C0 + 1 < TB) ? l2[(C0 + 1) * 4 + k] : double2{0.0, 0.0};
-#pragma unroll
-    for (int c = C0; c < TB; c += 2) {
+    for (int c = C0; c < C0 + NC; c += 2) {
         double2 u[4], v[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             u[k] = p0[k];
             v[k] = p1[k];
         }
-        if (c + 3 < TB)
+        if (c + 3 < C0 + NC)
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 p0[k] = l2[(c + 2) * 4 + k];
                 p1[k] = l2[(c + 3) * 4 + k];
             }
-        double s0 = a[c], s1 = a[c + 1];
+        double s0 = g[c], s1 = g[c + 1];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             s0 -= x[2 * k] * u[k].x;
@@ -219,138 +228,228 @@ __device__ __forceinline__ void rank8_update(double (&a)[TB], const double (&x)[
             s0 -= x[2 * k + 1] * u[k].y;
             s1 -= x[2 * k + 1] * v[k].y;
         }
-        a[c] = s0;
-        a[c + 1] = s1;
+        g[c] = s0;
+        g[c + 1] = s1;
         __builtin_amdgcn_sched_barrier(0);
     }
 }
 
-// Panel P0..P0+7 of the factor of a 64x64 tile held one row per lane in
-// a[] (entries right of the diagonal are ignored): the CPU algorithm of
-// func.cpp:806-836 in right-looking order -- pivot f (|f| and a count if
-// f < 0), column scaled by 1/sqrt(f) -- with the updates inside the panel
-// taken from the owning lanes by readlane, then the panel published to LP
-// and the rank-8 update of the columns to its right.  Lane j keeps 1/L_jj
-// in myinv.  Branch-free, so the row stays in registers.
-template <int P0>
-__device__ __forceinline__ void factor_panel8(double (&a)[TB], double *LP, double &myinv, int &nneg) {
+// Pivots of panel P (columns 8P..8P+7) held in g[L0 .. L0+7]; publishes the
+// panel to LP and the pivot inverses 1/L_jj of its lanes to ivs.
+// Branch-free, so the row stays in registers.
+template <int P, int L0>
+__device__ __forceinline__ void pivot_panel(double (&g)[16], double *LP, double *ivs, double &myinv, int &nneg) {
     const int lane = threadIdx.x & 63;
-    double *lp = LP + (P0 / 8) * (TB * 8);
 #pragma unroll
-    for (int j = P0; j < P0 + 8; j++) {
-        const double f0 = readlane_f64(a[j], j);
+    for (int j = 0; j < 8; j++) {
+        const int jc = 8 * P + j;
+        const double f0 = readlane_f64(g[L0 + j], jc);
         nneg += f0 < 0 ? 1 : 0;
         const double f = fabs(f0);
         const double inv = rsqrt_f64(f);
-        const double lj = lane > j ? a[j] * inv : (lane == j ? f * inv : 0.0);
-        myinv = lane == j ? inv : myinv;
-        a[j] = lj;
+        const double lj = lane > jc ? g[L0 + j] * inv : (lane == jc ? f * inv : 0.0);
+        myinv = lane == jc ? inv : myinv;
+        g[L0 + j] = lj;
 #pragma unroll
-        for (int c = j + 1; c < P0 + 8; c++) a[c] -= lj * readlane_f64(lj, c);
+        for (int c = j + 1; c < 8; c++) g[L0 + c] -= lj * readlane_f64(lj, 8 * P + c);
     }
+    double *lp = LP + P * (TB * 8);
 #pragma unroll
     for (int q = 0; q < 8; q += 2) {
         double2 v;
-        v.x = a[P0 + q];
-        v.y = a[P0 + q + 1];
+        v.x = g[L0 + q];
+        v.y = g[L0 + q + 1];
         *reinterpret_cast<double2 *>(lp + lane * 8 + q) = v;
     }
-    double x[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) x[q] = a[P0 + q];
-    __builtin_amdgcn_sched_barrier(0);
-    rank8_update<P0 + 8>(a, x, lp);
+    if (lane >= 8 * P && lane < 8 * P + 8) ivs[lane] = myinv;
 }
 
-// Panel P0..P0+7 of the row solve x L^T = b (row per lane in b[]) with the
-// factor in LP and the pivot inverses in iv.
-template <int P0>
-__device__ __forceinline__ void solve_panel8(double (&b)[TB], const double *LP, const double *iv) {
-    const double *lp = LP + (P0 / 8) * (TB * 8);
+// this lane's row of panel P, from LP
+template <int P>
+__device__ __forceinline__ void panel_row(const double *LP, double (&x)[8]) {
+    const double2 *r = reinterpret_cast<const double2 *>(LP + P * (TB * 8) + (threadIdx.x & 63) * 8);
 #pragma unroll
-    for (int j = P0; j < P0 + 8; j++) {
-        const double xj = b[j] * iv[j];
+    for (int k = 0; k < 4; k++) {
+        const double2 d = r[k];
+        x[2 * k] = d.x;
+        x[2 * k + 1] = d.y;
+    }
+}
+
+// X_ww = inv(L_ww) for the 16x16 diagonal block w, by wavefront w: lane r
+// (r < 16) solves x L_ww^T = e_r, whose solution is column r of X_ww.
+__device__ __forceinline__ void diag_block_inverse(const double *LP, const double *ivs, double *X, int w) {
+    const int lane = threadIdx.x & 63, r = lane & 15;
+    const double *lw = LP + w * (2 * TB * 8 + 16 * 8);  // L[16w + c][16w + j] = lw[(j/8)*512 + c*8 + j%8]
+    const double *iw = ivs + 16 * w;
+    double b[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) b[c] = (c == r) ? 1.0 : 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const double xj = b[j] * iw[j];
         b[j] = xj;
 #pragma unroll
-        for (int c = j + 1; c < P0 + 8; c++) b[c] -= xj * lp[c * 8 + (j - P0)];
+        for (int c = j + 1; c < 16; c++) b[c] -= xj * lw[(j >> 3) * (TB * 8) + c * 8 + (j & 7)];
     }
-    double x[8];
+    if (lane < 16)
 #pragma unroll
-    for (int q = 0; q < 8; q++) x[q] = b[P0 + q];
-    __builtin_amdgcn_sched_barrier(0);
-    rank8_update<P0 + 8>(b, x, lp);
+        for (int c = 0; c < 16; c++) X[(16 * w + c) * LDP + 16 * w + r] = b[c];
 }
 
-__device__ __forceinline__ void load_row(const double *A, int ld, int r0, int c0, double (&b)[TB]) {
-    const double *row = A + (size_t)(r0 + (threadIdx.x & 63)) * ld + c0;
+// 16x16 block products on the matrix cores (v_mfma_f64_16x16x4f64; result
+// register v of lane l is element (l/16 + 4v, l%16)), one wavefront each.
+// acc += L_ik X_kj, with L in panel order (LP) and X of pitch LDP.
+__device__ __forceinline__ dbl4 mm_lx(dbl4 acc, const double *LP, const double *X, int i, int k, int j) {
+    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
 #pragma unroll
-    for (int c = 0; c < TB; c += 2) {
-        const double2 v = *reinterpret_cast<const double2 *>(row + c);
-        b[c] = v.x;
-        b[c + 1] = v.y;
+    for (int kk = 0; kk < 16; kk += 4) {
+        const int col = 16 * k + kk + k4;
+        const double a = LP[(col >> 3) * (TB * 8) + (16 * i + i16) * 8 + (col & 7)];
+        const double b = X[(16 * k + kk + k4) * LDP + 16 * j + i16];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
+    return acc;
+}
+// acc += X_ii S, S a 16x16 block of pitch 17
+__device__ __forceinline__ dbl4 mm_xs(dbl4 acc, const double *X, const double *S, int i) {
+    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 4) {
+        const double a = X[(16 * i + i16) * LDP + 16 * i + kk + k4];
+        const double b = S[(kk + k4) * 17 + i16];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+// D (a 16x16 block of the given pitch) = sign * acc; the wavefront's later
+// LDS reads of D are ordered after these stores
+__device__ __forceinline__ void put_block(double *D, int pitch, dbl4 acc, double sign) {
+    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int v = 0; v < 4; v++) D[(k4 + 4 * v) * pitch + i16] = sign * acc[v];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+// X_ij = -X_ii S for S = sum_{k=j}^{i-1} L_ik X_kj (S: this wavefront's scratch)
+__device__ __forceinline__ void x_block(const double *LP, double *X, double *S, int i, int j) {
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k = j; k < i; k++) acc = mm_lx(acc, LP, X, i, k, j);
+    put_block(S, 17, acc, 1.0);
+    const dbl4 z = {0.0, 0.0, 0.0, 0.0};
+    put_block(X + 16 * i * LDP + 16 * j, LDP, mm_xs(z, X, S, i), -1.0);
+}
+// S = sum_{k=j}^{2} L_3k X_kj: the part of X_3j that does not need X_33
+__device__ __forceinline__ void s3_block(const double *LP, const double *X, double *S, int j) {
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k = j; k < 3; k++) acc = mm_lx(acc, LP, X, 3, k, j);
+    put_block(S, 17, acc, 1.0);
 }
 
-__device__ __forceinline__ void store_row(double *A, int ld, int r0, int c0, const double (&b)[TB], bool lower) {
-    const int lane = threadIdx.x & 63;
-    double *row = A + (size_t)(r0 + lane) * ld + c0;
+// One panel step of the factor (all four wavefronts; ends with a barrier).
+// Wavefronts without columns left build inv(L) meanwhile: X_00, X_11, X_22
+// as soon as their columns are final, X_10 (step 5), X_21, X_20 and
+// S_32 = L_32 X_22 (step 7); S: the wavefront's 16x17 scratch, S3: the
+// three partial sums S_3j.
+template <int P>
+__device__ __forceinline__ void factor_step(double (&g)[16], double *LP, double *ivs, double *X, double *S,
+                                            double *S3, double &myinv, int &nneg, int w) {
+    constexpr int O = P >> 1, LOC = P & 1;
+    if (w == O) {
+        if constexpr (LOC == 0 && P > 0) {
+            double x[8];
+            panel_row<P - 1>(LP, x);
+            __builtin_amdgcn_sched_barrier(0);
+            panel_update16<0, 16>(g, x, LP + (P - 1) * (TB * 8), 16 * w);
+        }
+        pivot_panel<P, 8 * LOC>(g, LP, ivs, myinv, nneg);
+        if constexpr (LOC == 0) {
+            double x[8];
 #pragma unroll
-    for (int c = 0; c < TB; c += 2) {
-        double2 v;
-        v.x = (!lower || c <= lane) ? b[c] : 0.0;
-        v.y = (!lower || c + 1 <= lane) ? b[c + 1] : 0.0;
-        *reinterpret_cast<double2 *>(row + c) = v;
+            for (int q = 0; q < 8; q++) x[q] = g[q];
+            __builtin_amdgcn_sched_barrier(0);
+            panel_update16<8, 8>(g, x, LP + P * (TB * 8), 16 * w);
+        }
+    } else if (w > O) {
+        if constexpr (P > 0) {
+            double x[8];
+            panel_row<P - 1>(LP, x);
+            __builtin_amdgcn_sched_barrier(0);
+            panel_update16<0, 16>(g, x, LP + (P - 1) * (TB * 8), 16 * w);
+        }
+    } else if (LOC == 0 && w == O - 1) {
+        diag_block_inverse(LP, ivs, X, w);
+    } else if (P == 5 && w == 0) {
+        x_block(LP, X, S, 1, 0);
+    } else if (P == 7) {
+        if (w == 0) s3_block(LP, X, S3 + 2 * (16 * 17), 2);
+        else if (w == 1) x_block(LP, X, S, 2, 1);
+        else x_block(LP, X, S, 2, 0);
     }
+    __syncthreads();
 }
 
-// Called by all four wavefronts of the workgroup.  Wavefront 0 factors the
-// tile T (LDS, pitch LDP) at (d0, d0) one panel of 8 columns at a time,
-// publishing each panel to LP (LDS scratch, 64*64) and its pivot inverses
-// to ivs (LDS, 64).  Wavefront 1 follows one panel behind and solves
-// x L^T = e_lane, which leaves column `lane` of inv(L) in its row: the
-// inverse costs one panel of latency beyond the factor.  L goes to A (lower,
-// zeros above) and inv(L), row-major 64x64, to Linv.
-__device__ __forceinline__ void diag_factor_inv(const double *T, double *LP, double *ivs, double *A, int ld,
-                                                int d0, double *Linv, int *neg) {
+// Called by all four wavefronts.  T: the tile (LDS, pitch LDP) at (d0, d0),
+// free for scratch once the rows are in registers (ivs and the per-wavefront
+// scratch of the inverse); LP: LDS, 64*64; X: LDS, 64*LDP.  L goes to A
+// (lower, zeros above) and inv(L), row-major 64x64, to Linv.
+__device__ __forceinline__ void diag_factor_inv(double *T, double *LP, double *X, double *A, int ld, int d0,
+                                                double *Linv, int *neg) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    double r[TB];
+    double g[16];
 #pragma unroll
-    for (int c = 0; c < TB; c++) r[c] = (w == 0) ? T[lane * LDP + c] : (c == lane ? 1.0 : 0.0);
+    for (int c = 0; c < 16; c++) g[c] = T[lane * LDP + 16 * w + c];
+    __syncthreads();
+    double *ivs = T, *S = T + TB + w * (16 * 17), *S3 = T + TB + 4 * (16 * 17);
     int nneg = 0;
     double myinv = 0.0;
-#define BD_DIAG_STEP(P)                                                        \
-    if (w == 0) {                                                              \
-        factor_panel8<P>(r, LP, myinv, nneg);                                  \
-        if (lane >= (P) && lane < (P) + 8) ivs[lane] = myinv;                  \
-    } else if (w == 1 && (P) > 0) {                                            \
-        solve_panel8<((P) > 0 ? (P) - 8 : 0)>(r, LP, ivs);                     \
-    }                                                                          \
-    __syncthreads();
-    BD_DIAG_STEP(0)
-    BD_DIAG_STEP(8)
-    BD_DIAG_STEP(16)
-    BD_DIAG_STEP(24)
-    BD_DIAG_STEP(32)
-    BD_DIAG_STEP(40)
-    BD_DIAG_STEP(48)
-    BD_DIAG_STEP(56)
-#undef BD_DIAG_STEP
-    if (w == 1) {
-        solve_panel8<56>(r, LP, ivs);
+    factor_step<0>(g, LP, ivs, X, S, S3, myinv, nneg, w);
+    factor_step<1>(g, LP, ivs, X, S, S3, myinv, nneg, w);
+    factor_step<2>(g, LP, ivs, X, S, S3, myinv, nneg, w);
+    factor_step<3>(g, LP, ivs, X, S, S3, myinv, nneg, w);
+    factor_step<4>(g, LP, ivs, X, S, S3, myinv, nneg, w);
+    factor_step<5>(g, LP, ivs, X, S, S3, myinv, nneg, w);
+    factor_step<6>(g, LP, ivs, X, S, S3, myinv, nneg, w);
+    factor_step<7>(g, LP, ivs, X, S, S3, myinv, nneg, w);
+    // L: each wavefront its 16 columns (zeros above the diagonal: a lane
+    // above a pivot column takes 0 at that pivot)
+    double *row = A + (size_t)(d0 + lane) * ld + d0 + 16 * w;
 #pragma unroll
-        for (int c = 0; c < TB; c++) Linv[c * TB + lane] = r[c];
-    } else if (w == 0) {
-        store_row(A, ld, d0, d0, r, true);
-        if (nneg && lane == 0) atomicAdd(neg, nneg);
+    for (int c = 0; c < 16; c += 2) {
+        dbl2 v;
+        v.x = g[c];
+        v.y = g[c + 1];
+        *reinterpret_cast<dbl2 *>(row + c) = v;
+    }
+    if (nneg && lane == 0) atomicAdd(neg, nneg);
+    // X_33 beside the partial sums S_30, S_31; then X_3j = -X_33 S_3j
+    if (w == 3) diag_block_inverse(LP, ivs, X, 3);
+    else if (w == 0) s3_block(LP, X, S3, 0);
+    else if (w == 1) s3_block(LP, X, S3 + 16 * 17, 1);
+    __syncthreads();
+    if (w < 3) {
+        const dbl4 z = {0.0, 0.0, 0.0, 0.0};
+        put_block(X + 48 * LDP + 16 * w, LDP, mm_xs(z, X, S3 + w * (16 * 17), 3), -1.0);
+    }
+    __syncthreads();
+    // inv(L), zeros in the blocks above the diagonal
+#pragma unroll
+    for (int i = 0; i < TB * TB / 2 / NTHR; i++) {
+        const int e = threadIdx.x + i * NTHR, r = e / (TB / 2), c = 2 * (e % (TB / 2));
+        dbl2 v;
+        const bool keep = (r >> 4) >= (c >> 4);
+        v.x = keep ? X[r * LDP + c] : 0.0;
+        v.y = keep ? X[r * LDP + c + 1] : 0.0;
+        *reinterpret_cast<dbl2 *>(Linv + r * TB + c) = v;
     }
 }
 
 // first diagonal tile
 __global__ void __launch_bounds__(256) k_potrf0(double *A, int ld, double *Linv, int *neg) {
-    __shared__ double T[TB * LDP], LP[TB * TB], ivs[TB];
+    __shared__ double T[TB * LDP], LP[TB * TB], X[TB * LDP];
     load_tile(A, ld, 0, 0, T);
     __syncthreads();
-    diag_factor_inv(T, LP, ivs, A, ld, 0, Linv, neg);
+    diag_factor_inv(T, LP, X, A, ld, 0, Linv, neg);
 }
 
 // One block column kb of the factorisation in one launch, one workgroup per

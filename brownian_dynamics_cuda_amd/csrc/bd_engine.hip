@@ -67,6 +67,8 @@ struct bd_context {
     // predictor translation
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // branches of the two force passes of a step (streams shared, events per pass)
+    ForceStreams fstr[2]{};
     std::vector<void *> allocs;
     // sizes
     int n = 0, n_c = 0, n_tail = 0, n_LH = 0, lh_cap = 0, n_D = 0, n_D3 = 0, n_hi = 0, hi_all = 0;
@@ -166,8 +168,12 @@ struct Marker {
 };
 
 // The force pass at configuration g into f (energy only when Epart given).
-void enqueue_forces(bd_context *c, const Geo &g, const Frc &f, double *Epart, int terms) {
-    launch_forces(c->st, g, c->sites, f, c->fw, Epart, terms, c->stream, c->timing ? mark_cb : nullptr, c);
+// Pass 2 (pass == 1) runs its branches on their own streams unless the
+// timing markers are on; pass 1 stays on one stream, because it already
+// shares the GPU with the factorisation on the side stream.
+void enqueue_forces(bd_context *c, const Geo &g, const Frc &f, double *Epart, int terms, int pass) {
+    launch_forces(c->st, g, c->sites, f, c->fw, Epart, terms, c->stream, c->timing ? mark_cb : nullptr, c,
+                  (c->timing || pass == 0) ? nullptr : &c->fstr[pass]);
 }
 
 void enqueue_factor(bd_context *c, const Geo &g, hipStream_t q) {
@@ -206,7 +212,7 @@ void enqueue_step(bd_context *c, bool refactor) {
     // LH bending angles at the current state (3497)
     M(K_BETAS, 0); launch_betas(s, c->cur, 0, 1, q); M(K_BETAS, 1);
     // force pass 1 at x0 and its energy (3556-3615)
-    enqueue_forces(c, c->cur, c->f0, c->Epart, BD_TERM_ALL);
+    enqueue_forces(c, c->cur, c->f0, c->Epart, BD_TERM_ALL, 0);
     M(K_EREDUCE, 0); launch_energy_reduce(c->Epart, c->ne, c->energy, q); M(K_EREDUCE, 1);
     // predictor (3617-3619)
     M(K_ROTATE, 0); launch_rotate(s, c->cur, c->pred, c->f0.T, c->rr, q); M(K_ROTATE, 1);
@@ -219,7 +225,7 @@ void enqueue_step(bd_context *c, bool refactor) {
     M(K_BETAS, 0); launch_betas(s, c->pred, 1, 1, q); M(K_BETAS, 1);
     M(K_FRAMES, 0); launch_frames(s, c->pred, q); M(K_FRAMES, 1);
     // force pass 2 (3628-3646)
-    enqueue_forces(c, c->pred, c->f1, nullptr, BD_TERM_ALL);
+    enqueue_forces(c, c->pred, c->f1, nullptr, BD_TERM_ALL, 1);
     // corrector from x0 with the averaged forces, same rd and rr (3648-3652)
     M(K_AVG, 0); launch_average(s, c->f0, c->f1, c->fa, q); M(K_AVG, 1);
     M(K_ROTATE, 0); launch_rotate(s, c->cur, c->pred, c->fa.T, c->rr, q); M(K_ROTATE, 1);
@@ -330,6 +336,17 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
         return bail(fail(BD_E_HIP, "hipEventCreate failed"));
+    for (int i = 0; i < 3; i++) {
+        hipStream_t b = nullptr;
+        if (hipStreamCreateWithFlags(&b, hipStreamNonBlocking) != hipSuccess)
+            return bail(fail(BD_E_HIP, "hipStreamCreate failed"));
+        c->fstr[0].b[i] = c->fstr[1].b[i] = b;
+    }
+    for (auto &fs : c->fstr) {
+        bool ok = hipEventCreateWithFlags(&fs.fork, hipEventDisableTiming) == hipSuccess;
+        for (auto &j : fs.join) ok = ok && hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess;
+        if (!ok) return bail(fail(BD_E_HIP, "hipEventCreate failed"));
+    }
 
     // host-side index tables
     std::vector<int> ic_of(n), core_bead, elig(n), lnk, lhc_bead, lh_of_core(n_c, -1);
@@ -500,6 +517,16 @@ int bd_destroy(bd_context *c) {
     if (c->batch) (void)hipFree(c->batch);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    for (int i = 0; i < 3; i++)
+        if (c->fstr[0].b[i]) {
+            (void)hipStreamSynchronize(c->fstr[0].b[i]);
+            (void)hipStreamDestroy(c->fstr[0].b[i]);
+        }
+    for (auto &fs : c->fstr) {
+        if (fs.fork) (void)hipEventDestroy(fs.fork);
+        for (auto &j : fs.join)
+            if (j) (void)hipEventDestroy(j);
+    }
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -718,7 +745,7 @@ int bd_forces(bd_context *c, int terms, double *F, double *T, double *tF, double
     if (check(c)) return BD_E_ARG;
     if (!c->state_set) return fail(BD_E_STATE, "bd_set_state must be called first");
     HIPCHK(hipSetDevice(c->device));
-    enqueue_forces(c, c->cur, c->f0, c->Epart, terms);
+    enqueue_forces(c, c->cur, c->f0, c->Epart, terms, 0);
     launch_energy_reduce(c->Epart, c->ne, c->energy, c->stream);
     HIPCHK(hipGetLastError());
     if (c->timing) collect_timing(c);

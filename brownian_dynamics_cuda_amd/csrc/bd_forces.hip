@@ -863,28 +863,45 @@ void launch_sites(const Stat &s, const Geo &g, const Sites &st, hipStream_t q) {
 }
 
 void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, const ForceScratch &w,
-                   double *Epart, int terms, hipStream_t q, MarkFn mark, void *arg) {
+                   double *Epart, int terms, hipStream_t q, MarkFn mark, void *arg, const ForceStreams *fs) {
     const int off_mech = 0, off_lnk = s.n, off_tail = off_lnk + s.n_lnk, off_lh = off_tail + s.n_tail,
               off_core = off_lh + s.n_LH, off_cc = off_core + s.n_c;
     auto E = [&](int off) { return Epart ? Epart + off : nullptr; };
     auto M = [&](int id, int ph) { if (mark) mark(arg, id, ph); };
     M(K_SITES, 0); launch_sites(s, g, st, q); M(K_SITES, 1);
-    M(K_MECH, 0); k_mech<<<nblk(s.n, 64), 64, 0, q>>>(s, g, f, E(off_mech), terms); M(K_MECH, 1);
-    if (s.n_lnk) { M(K_LINKER, 0); k_linker<<<s.n_lnk, kRecvThreads, 0, q>>>(s, g, st, f, E(off_lnk), terms); M(K_LINKER, 1); }
-    if (s.n_tail) { M(K_TAIL, 0); k_tail<<<s.n_tail, kRecvThreads, 0, q>>>(s, g, st, f, E(off_tail), terms); M(K_TAIL, 1); }
-    if (s.n_LH) { M(K_LH, 0); k_lh<<<s.n_LH, kRecvThreads, 0, q>>>(s, g, st, f, E(off_lh), terms); M(K_LH, 1); }
+    // k_mech assigns F, T of every main bead and k_linker then adds to the
+    // linker beads; k_tail and k_lh assign their own arrays; k_core adds to
+    // the cores after k_mech and gathers k_cc_work's slots
+    hipStream_t qt = q, ql = q, qc = q;
+    if (fs) {
+        (void)hipEventRecord(fs->fork, q);
+        for (int i = 0; i < 3; i++) (void)hipStreamWaitEvent(fs->b[i], fs->fork, 0);
+        qt = fs->b[0];
+        ql = fs->b[1];
+        qc = fs->b[2];
+    }
+    if (s.n_tail) { M(K_TAIL, 0); k_tail<<<s.n_tail, kRecvThreads, 0, qt>>>(s, g, st, f, E(off_tail), terms); M(K_TAIL, 1); }
+    if (s.n_LH) { M(K_LH, 0); k_lh<<<s.n_LH, kRecvThreads, 0, ql>>>(s, g, st, f, E(off_lh), terms); M(K_LH, 1); }
     if (terms & TERM_ELEC) {
         long long nn = (long long)s.n_c * s.n_c;
         M(K_CCLIST, 0);
-        k_fill_i<<<1, 64, 0, q>>>(w.cc_count, 1, 0);
-        k_cc_list<<<nblk(nn, 256), 256, 0, q>>>(s, g, w.cc_valid, w.cc_list, w.cc_count);
+        k_fill_i<<<1, 64, 0, qc>>>(w.cc_count, 1, 0);
+        k_cc_list<<<nblk(nn, 256), 256, 0, qc>>>(s, g, w.cc_valid, w.cc_list, w.cc_count);
         M(K_CCLIST, 1);
         M(K_CCWORK, 0);
         if (Epart)
-            k_cc_work<true><<<w.cc_grid, kCCThreads, 0, q>>>(s, g, st, w.cc_list, w.cc_count, w.cc_nchunk, w.cc_slots);
+            k_cc_work<true><<<w.cc_grid, kCCThreads, 0, qc>>>(s, g, st, w.cc_list, w.cc_count, w.cc_nchunk, w.cc_slots);
         else
-            k_cc_work<false><<<w.cc_grid, kCCThreads, 0, q>>>(s, g, st, w.cc_list, w.cc_count, w.cc_nchunk, w.cc_slots);
+            k_cc_work<false><<<w.cc_grid, kCCThreads, 0, qc>>>(s, g, st, w.cc_list, w.cc_count, w.cc_nchunk, w.cc_slots);
         M(K_CCWORK, 1);
+    }
+    M(K_MECH, 0); k_mech<<<nblk(s.n, 64), 64, 0, q>>>(s, g, f, E(off_mech), terms); M(K_MECH, 1);
+    if (s.n_lnk) { M(K_LINKER, 0); k_linker<<<s.n_lnk, kRecvThreads, 0, q>>>(s, g, st, f, E(off_lnk), terms); M(K_LINKER, 1); }
+    if (fs) {
+        for (int i = 0; i < 3; i++) {
+            (void)hipEventRecord(fs->join[i], fs->b[i]);
+            (void)hipStreamWaitEvent(q, fs->join[i], 0);
+        }
     }
     M(K_CORE, 0);
     k_core<<<s.n_c, kCoreThreads, 0, q>>>(s, g, st, f, w.cc_valid, w.cc_nchunk, w.cc_slots, E(off_core), E(off_cc), terms);

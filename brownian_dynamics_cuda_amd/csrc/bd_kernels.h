@@ -22,8 +22,16 @@ enum KernelId { K_SITES, K_MECH, K_LINKER, K_TAIL, K_LH, K_CORE, K_CCLIST, K_CCW
                 K_RPY, K_POTRF, K_PANEL_COPY, K_CHOL_STEP, K_NOISE, K_FRAMES, K_BETAS, K_ROTATE, K_TRANSLATE,
                 K_AVG, K_COMMIT, K_EREDUCE, K_COUNT };
 typedef void (*MarkFn)(void *, int, int);
+// Extra streams of a force pass: after the charge sites, the tail, the
+// linker-histone and the core-core branches run beside mechanics + linker
+// on the pass's stream, and the core kernel waits for all of them.
+struct ForceStreams {
+    hipStream_t b[3];
+    hipEvent_t fork, join[3];
+};
 void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, const ForceScratch &w,
-                   double *Epart, int terms, hipStream_t q, MarkFn mark = nullptr, void *arg = nullptr);
+                   double *Epart, int terms, hipStream_t q, MarkFn mark = nullptr, void *arg = nullptr,
+                   const ForceStreams *fs = nullptr);
 int energy_slots(const Stat &s);
 void launch_energy_reduce(const double *Epart, int ne, double *out, hipStream_t q);
 void launch_fill(double *p, long long n, double v, hipStream_t q);
