@@ -559,7 +559,7 @@ __global__ void __launch_bounds__(256) k_panel_copy(double *A, int ld) {
     }
 }
 
-void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_arg) {
+void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_arg, hipEvent_t gate, int gate_kb) {
     const int nt = c.mpad / TB;
     auto M_ = [&](int id, int ph) { if (mark) mark(mark_arg, id, ph); };
     launch_fill_int(c.neg, 1, 0, q);
@@ -574,7 +574,9 @@ void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_a
         else
             k_step<1><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
         M_(K_CHOL_STEP, 1);
+        if (gate && kb == gate_kb) (void)hipEventRecord(gate, q);
     }
+    if (gate && gate_kb >= nt - 1) (void)hipEventRecord(gate, q);
     if (nt > 1) {
         M_(K_PANEL_COPY, 0);
         k_panel_copy<<<nt * (nt - 1) / 2, 256, 0, q>>>(c.A, c.ld);

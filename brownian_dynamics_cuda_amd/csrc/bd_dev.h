@@ -72,9 +72,17 @@ struct Stat {
     double rmax_g;    // max |LH_g_pos|
 };
 
+// partners per bounding sphere of the culled partner loops
+constexpr int kChunk = 32;
+__host__ __device__ inline int nchunks(int n) { return (n + kChunk - 1) / kChunk; }
+
 struct Sites {
     double *Q;  // 3 * n_c * Nq world positions of core charges
     double *G;  // 3 * n_lhc * n_lh_g world positions of globular-head sites
+    // bounding spheres (centre x, y, z, radius) of kChunk consecutive
+    // partners, rebuilt per pass: tail beads, free LH beads, head sites, and
+    // linker beads in the order of the linker list
+    double4 *ch_t, *ch_lh, *ch_g, *ch_l;
 };
 
 __device__ __forceinline__ double3 ld3(const double *p) { return make_double3(p[0], p[1], p[2]); }
@@ -96,6 +104,30 @@ __device__ __forceinline__ double rsqrt_f64(double f) {
     const double y = __builtin_amdgcn_rsq(f);
     const double r = __builtin_fma(-f * y, y, 1.0);
     return __builtin_fma(y * r, __builtin_fma(r, 0.375, 0.5), y);
+}
+// e^x for -700 < x <= 0 without range checks: x = n ln2 + r with
+// |r| <= ln2/2 (n rounded to nearest; ln2 split into a high part with
+// trailing zero bits and the low remainder, the published fdlibm constants,
+// so that n ln2_hi is exact), e^r by its Taylor series to r^12 (truncation
+// below 2e-16 relative) in Horner form, scaled by 2^n.
+__device__ __forceinline__ double exp_neg(double x) {
+    const double n = rint(x * 1.4426950408889634);
+    double r = __builtin_fma(-n, 6.93147180369123816490e-01, x);
+    r = __builtin_fma(-n, 1.90821492927058770002e-10, r);
+    double p = 1.0 / 479001600.0;
+    p = __builtin_fma(p, r, 1.0 / 39916800.0);
+    p = __builtin_fma(p, r, 1.0 / 3628800.0);
+    p = __builtin_fma(p, r, 1.0 / 362880.0);
+    p = __builtin_fma(p, r, 1.0 / 40320.0);
+    p = __builtin_fma(p, r, 1.0 / 5040.0);
+    p = __builtin_fma(p, r, 1.0 / 720.0);
+    p = __builtin_fma(p, r, 1.0 / 120.0);
+    p = __builtin_fma(p, r, 1.0 / 24.0);
+    p = __builtin_fma(p, r, 1.0 / 6.0);
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    return __builtin_ldexp(p, (int)n);
 }
 __device__ __forceinline__ double3 zero3() { return make_double3(0.0, 0.0, 0.0); }
 

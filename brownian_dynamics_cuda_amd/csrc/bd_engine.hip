@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -66,7 +67,10 @@ struct bd_context {
     // force pass 1 (both read only the state at x0); joined before the
     // predictor translation
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_gate = nullptr;
+    // force pass 1 waits for the factor to pass this block column (-1: the
+    // default fraction of the block columns; BD_GATE_KB overrides it)
+    int gate_kb = -1;
     // branches of the two force passes of a step (streams shared, events per pass)
     ForceStreams fstr[2]{};
     std::vector<void *> allocs;
@@ -176,12 +180,12 @@ void enqueue_forces(bd_context *c, const Geo &g, const Frc &f, double *Epart, in
                   (c->timing || pass == 0) ? nullptr : &c->fstr[pass]);
 }
 
-void enqueue_factor(bd_context *c, const Geo &g, hipStream_t q) {
+void enqueue_factor(bd_context *c, const Geo &g, hipStream_t q, hipEvent_t gate = nullptr, int gate_kb = -1) {
     Marker M{c};
     M(K_RPY, 0);
     launch_rpy(c->st, c->n_hi, g, c->rad_all, c->chol, q);
     M(K_RPY, 1);
-    launch_cholesky(c->chol, q, M.fn(), c);
+    launch_cholesky(c->chol, q, M.fn(), c, gate, gate_kb);
 }
 
 void drop_graphs(bd_context *c) {
@@ -207,8 +211,15 @@ void enqueue_step(bd_context *c, bool refactor) {
         (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
         qf = c->side;
     }
-    if (refactor) enqueue_factor(c, c->cur, qf);
+    // force pass 1 starts once the factor is past block column gate_kb:
+    // the early block columns fill the GPU, and force kernels beside them
+    // hold the CUs that the diagonal-tile workgroups wait for
+    const int nt = c->chol.mpad / 64;
+    const int gate_kb = c->gate_kb >= 0 ? c->gate_kb : (nt - 1) * 45 / 100;
+    const bool gated = refactor && qf != q;
+    if (refactor) enqueue_factor(c, c->cur, qf, gated ? c->ev_gate : nullptr, gate_kb);
     M(K_NOISE, 0); launch_noise(s, c->n_hi, c->n_D3, c->chol, c->p, c->rad_all, c->rd, qf); M(K_NOISE, 1);
+    if (gated) (void)hipStreamWaitEvent(q, c->ev_gate, 0);
     // LH bending angles at the current state (3497)
     M(K_BETAS, 0); launch_betas(s, c->cur, 0, 1, q); M(K_BETAS, 1);
     // force pass 1 at x0 and its energy (3556-3615)
@@ -320,6 +331,7 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     bd_context *c = new bd_context();
     c->device = o.device;
     c->use_graph = o.use_graph != 0;
+    if (const char *gk = std::getenv("BD_GATE_KB")) c->gate_kb = std::atoi(gk);
     c->n = n; c->n_c = n_c; c->n_tail = nt; c->n_LH = sys->n_LH; c->nlh = nlh;
     c->lh_cap = std::max(sys->n_LH, o.lh_capacity);
     c->hi_all = sys->hi_all ? 1 : 0;
@@ -334,7 +346,8 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
         hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(BD_E_HIP, "hipStreamCreate failed"));
     if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_gate, hipEventDisableTiming) != hipSuccess)
         return bail(fail(BD_E_HIP, "hipEventCreate failed"));
     for (int i = 0; i < 3; i++) {
         hipStream_t b = nullptr;
@@ -454,6 +467,10 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     if (alloc_frc(c, c->f0) || alloc_frc(c, c->f1) || alloc_frc(c, c->fa)) return bail(BD_E_HIP);
     r |= c->alloc(&c->sites.Q, 3 * (size_t)n_c * Nq);
     r |= c->alloc(&c->sites.G, 3 * (size_t)n_c * std::max(sys->n_lh_g, 1));
+    r |= c->alloc(&c->sites.ch_t, nchunks(nt));
+    r |= c->alloc(&c->sites.ch_lh, nchunks(c->lh_cap));
+    r |= c->alloc(&c->sites.ch_g, nchunks(n_c * std::max(sys->n_lh_g, 1)));
+    r |= c->alloc(&c->sites.ch_l, nchunks(n));
     // core-core scratch
     c->fw.cc_nchunk = 2;
     while ((Nq + c->fw.cc_nchunk - 1) / c->fw.cc_nchunk > 160) c->fw.cc_nchunk++;
@@ -465,6 +482,9 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
         r |= c->alloc(&c->fw.cc_list, 2 * (size_t)std::max(npairs, 1LL));
         r |= c->alloc(&c->fw.cc_count, 1);
         r |= c->alloc(&c->fw.cc_slots, (size_t)n_c * n_c * c->fw.cc_nchunk * 10);
+        c->fw.core_split = 4;
+        r |= c->alloc(&c->fw.core_part, (size_t)n_c * c->fw.core_split * 8);
+        r |= c->alloc(&c->fw.core_cnt, n_c);
     }
     // Cholesky workspace
     c->chol.m = 3 * c->n_hi;
@@ -487,6 +507,7 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     if (r) return bail(BD_E_HIP);
     launch_fill(c->rr, 3 * n, 0.0, c->stream);
     launch_fill(c->p, 3 * nD_cap, 0.0, c->stream);
+    launch_fill_int(c->fw.core_cnt, n_c, 0, c->stream);
     // entries the step never writes (e.g. the exit frame of a final core)
     // read as 0, like the host arrays of the reference driver
     for (Geo *g : {&c->cur, &c->pred}) {
@@ -517,6 +538,7 @@ int bd_destroy(bd_context *c) {
     if (c->batch) (void)hipFree(c->batch);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->ev_gate) (void)hipEventDestroy(c->ev_gate);
     for (int i = 0; i < 3; i++)
         if (c->fstr[0].b[i]) {
             (void)hipStreamSynchronize(c->fstr[0].b[i]);

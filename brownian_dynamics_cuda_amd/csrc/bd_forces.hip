@@ -131,6 +131,114 @@ __device__ __forceinline__ double3 torque_abs(double3 f, double3 pf, double3 pc,
 }
 
 // Thread per main bead j: assigns F[j], T[j] (this kernel opens the pass).
+// Bounding sphere of chunk c of kChunk consecutive partners (centroid and
+// the largest distance from it), by one wavefront, a member per lane; idx
+// maps a partner to its bead (NULL: the partner is the bead).  Returns the
+// sphere in every lane.
+__device__ __forceinline__ double4 chunk_sphere(const double *x, const int *idx, int c, int n) {
+    const int lane = threadIdx.x & 63, k = c * kChunk + lane;
+    const int cnt = min(n - c * kChunk, kChunk);
+    const bool in = lane < cnt;
+    const double3 p = in ? ld3(x + 3 * (idx ? idx[k] : k)) : zero3();
+    double3 m = p;
+    for (int o = 32; o > 0; o >>= 1) {
+        m.x += __shfl_xor(m.x, o, 64);
+        m.y += __shfl_xor(m.y, o, 64);
+        m.z += __shfl_xor(m.z, o, 64);
+    }
+    const double w = 1.0 / cnt;
+    m = make_double3(m.x * w, m.y * w, m.z * w);
+    const double3 d = p - m;
+    double r2 = in ? dot(d, d) : 0.0;
+    for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+    return make_double4(m.x, m.y, m.z, sqrt(r2));
+}
+
+// Chunk spheres of the tail beads, free LH beads, head sites and linker
+// beads, one wavefront per chunk.
+__global__ void __launch_bounds__(256) k_chunks(Stat s, Geo g, Sites st) {
+    int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const bool w0 = (threadIdx.x & 63) == 0;
+    const int a = nchunks(s.n_tail), b = nchunks(s.n_LH), c = nchunks(s.n_lhc * s.n_lh_g);
+    if (t < a) { const double4 h = chunk_sphere(g.r_t, nullptr, t, s.n_tail); if (w0) st.ch_t[t] = h; return; }
+    t -= a;
+    if (t < b) { const double4 h = chunk_sphere(g.r_lh, nullptr, t, s.n_LH); if (w0) st.ch_lh[t] = h; return; }
+    t -= b;
+    if (t < c) { const double4 h = chunk_sphere(st.G, nullptr, t, s.n_lhc * s.n_lh_g); if (w0) st.ch_g[t] = h; return; }
+    t -= c;
+    if (t < nchunks(s.n_lnk)) { const double4 h = chunk_sphere(g.r, s.lnk, t, s.n_lnk); if (w0) st.ch_l[t] = h; }
+}
+
+// Ordered stream compaction over an NT-thread workgroup: threads with keep
+// write value to list in thread order; returns the count.  Contains
+// barriers: every thread of the workgroup calls it.
+template <int NT>
+__device__ __forceinline__ int compact_ordered(bool keep, int value, int *list, int *wsum) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(keep);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; i++) {
+        const int c = wsum[i];
+        off += (i < w) ? c : 0;
+        tot += c;
+    }
+    if (keep) list[off + before] = value;
+    __syncthreads();
+    return tot;
+}
+
+// The partners k < n within reach of the sphere (p, rp): the chunks whose
+// sphere comes within reach are compacted in index order, and their members
+// are dealt out in that order, member f of the list going to the flat thread
+// index f (mod stride) -- start is this thread's flat index -- so every
+// thread visits a fixed sequence: fn(k).  Partners left out are beyond the
+// cut-off of every term that uses this (7 nm), where the terms are exactly 0.
+// Contains barriers: every thread of the workgroup calls it.
+constexpr double kReach = 7.0 + 1e-6;
+template <int NT, typename Fn>
+__device__ __forceinline__ void for_near(const double4 *ch, int n, double3 p, double rp, int start, int stride,
+                                         int *list, int *wsum, Fn &&fn) {
+    const int nch = nchunks(n);
+    for (int base = 0; base < nch; base += NT) {
+        const int c = base + threadIdx.x;
+        bool keep = false;
+        if (c < nch) {
+            const double4 h = ch[c];
+            const double3 d = make_double3(p.x - h.x, p.y - h.y, p.z - h.z);
+            keep = !(sqrt(dot(d, d)) - h.w - rp > kReach);
+        }
+        const int cnt = compact_ordered<NT>(keep, c, list, wsum);
+        for (int f = start; f < cnt * kChunk; f += stride) {
+            const int k = list[f / kChunk] * kChunk + f % kChunk;
+            if (k < n) fn(k);
+        }
+        __syncthreads();
+    }
+}
+
+// Sphere around a set of points (centroid, largest distance from it).
+template <int N>
+__device__ __forceinline__ void points_sphere(const double3 (&x)[N], int n, double3 &c, double &r) {
+    c = zero3();
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        if (i < n) c += x[i];
+    const double w = 1.0 / n;
+    c = make_double3(c.x * w, c.y * w, c.z * w);
+    double r2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        if (i < n) {
+            const double3 d = x[i] - c;
+            r2 = fmax(r2, dot(d, d));
+        }
+    r = sqrt(r2);
+}
+
 __global__ void k_mech(Stat s, Geo g, Frc f, double *Epart, int terms) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= s.n) return;
@@ -227,6 +335,7 @@ constexpr int kRecvThreads = 256;
 // F[j] +=.  Energy owner: DNA-DNA pairs with partner index > j.
 __global__ void __launch_bounds__(kRecvThreads) k_linker(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
     __shared__ double red[4 * (kRecvThreads / 64)];
+    __shared__ int s_list[kRecvThreads], s_wsum[kRecvThreads / 64];
     const int li = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
     const int j = s.lnk[li];
     const double3 rj = ld3(g.r + 3 * j);
@@ -268,18 +377,21 @@ __global__ void __launch_bounds__(kRecvThreads) k_linker(Stat s, Geo g, Sites st
         }
     }
     if (terms & TERM_TAIL)  // tail-linker, energy owned by tails
-        for (int t = tid; t < s.n_tail; t += nth)
+        for_near<kRecvThreads>(st.ch_t, s.n_tail, rj, 0.0, tid, nth, s_list, s_wsum, [&](int t) {
             dhev_recv(s.t_q[t] * s.q_l, s.c4pe, s.debye, s.k_ex, s.sig_tl, rj, ld3(g.r_t + 3 * t), F, Ed);
+        });
     if (terms & TERM_LH) {
-        for (int i = tid; i < s.n_LH; i += nth) {  // LH-DNA, energy owned by LH beads
+        // LH-DNA, energy owned by LH beads
+        for_near<kRecvThreads>(st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int i) {
             int idx = i % s.nlh;
             idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
             dhev_recv(s.LH_q[idx] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[idx], rj, ld3(g.r_lh + 3 * i), F, Ed);
-        }
-        for (int x = tid; x < s.n_lhc * s.n_lh_g; x += nth) {  // LHG-DNA, energy on the core side
+        });
+        // LHG-DNA, energy on the core side
+        for_near<kRecvThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
             int m = x % s.n_lh_g;
             dhev_recv(s.LH_q[m] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[m], rj, ld3(st.G + 3 * x), F, Ed);
-        }
+        });
     }
     double v[4] = {F.x, F.y, F.z, E};
     block_sum_n<kRecvThreads, 4>(v, red);
@@ -295,6 +407,7 @@ __global__ void __launch_bounds__(kRecvThreads) k_linker(Stat s, Geo g, Sites st
 // partner > j, tail-linker pairs, the anchor spring.
 __global__ void __launch_bounds__(kRecvThreads) k_tail(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
     __shared__ double red[4 * (kRecvThreads / 64)];
+    __shared__ int s_list[kRecvThreads], s_wsum[kRecvThreads / 64];
     const int j = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
     const int nt = s.n_tail;
     const int *tg = s.t_grp;
@@ -328,26 +441,27 @@ __global__ void __launch_bounds__(kRecvThreads) k_tail(Stat s, Geo g, Sites st, 
         }
         // tail-tail DH (1863-1891), skipping same-group neighbours
         const double qj = s.t_q[j];
-        for (int k = tid; k < nt; k += nth) {
-            if (k == j) continue;
-            if (!(tg[k] != tg[j] || abs(k - j) > 2)) continue;
+        for_near<kRecvThreads>(st.ch_t, nt, rj, 0.0, tid, nth, s_list, s_wsum, [&](int k) {
+            if (k == j) return;
+            if (!(tg[k] != tg[j] || abs(k - j) > 2)) return;
             dh_recv(qj * s.t_q[k], s.c4pe, s.debye, rj, ld3(g.r_t + 3 * k), F, (k > j) ? E : Ed);
-        }
+        });
         // tail-linker (1898-1917)
-        for (int pi = tid; pi < s.n_lnk; pi += nth)
+        for_near<kRecvThreads>(st.ch_l, s.n_lnk, rj, 0.0, tid, nth, s_list, s_wsum, [&](int pi) {
             dhev_recv(qj * s.q_l, s.c4pe, s.debye, s.k_ex, s.sig_tl, rj, ld3(g.r + 3 * s.lnk[pi]), F, E);
+        });
     }
     if ((terms & TERM_LH) && s.t_fix[j] == 0) {
         // LH-tail and LHG-tail, tail side; anchored tails pass these to a core (k_core)
-        for (int i = tid; i < s.n_LH; i += nth) {
+        for_near<kRecvThreads>(st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int i) {
             int idx = i % s.nlh;
             idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
             dhev_recv(s.LH_q[idx] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rj, ld3(g.r_lh + 3 * i), F, Ed);
-        }
-        for (int x = tid; x < s.n_lhc * s.n_lh_g; x += nth) {
+        });
+        for_near<kRecvThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
             int m = x % s.n_lh_g;
             dhev_recv(s.LH_q[m] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], rj, ld3(st.G + 3 * x), F, Ed);
-        }
+        });
     }
     double v[4] = {F.x, F.y, F.z, E};
     block_sum_n<kRecvThreads, 4>(v, red);
@@ -361,6 +475,7 @@ __global__ void __launch_bounds__(kRecvThreads) k_tail(Stat s, Geo g, Sites st, 
 // owner: bonds / angles at j, LH-LH pairs with partner > j, LH-DNA, LH-tail.
 __global__ void __launch_bounds__(kRecvThreads) k_lh(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
     __shared__ double red[4 * (kRecvThreads / 64)];
+    __shared__ int s_list[kRecvThreads], s_wsum[kRecvThreads / 64];
     const int j = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
     const int nlh = s.nlh;
     auto tpl = [&](int i) { int x = i % nlh; return (x < s.n_lh_n) ? x : x + s.n_lh_g; };
@@ -403,8 +518,8 @@ __global__ void __launch_bounds__(kRecvThreads) k_lh(Stat s, Geo g, Sites st, Fr
             }
         }
         // LH-LH (2190-2244)
-        for (int k = tid; k < s.n_LH; k += nth) {
-            if (k == j) continue;
+        for_near<kRecvThreads>(st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int k) {
+            if (k == j) return;
             int i2 = tpl(k);
             double sg = (s.LH_hh[idx] + s.LH_hh[i2]) / 2;
             double &Ek = (k > j) ? E : Ed;
@@ -413,20 +528,22 @@ __global__ void __launch_bounds__(kRecvThreads) k_lh(Stat s, Geo g, Sites st, Fr
             } else {
                 dhev_recv(s.LH_q[idx] * s.LH_q[i2], s.c4pe, s.debye, s.k_ex, sg, rj, ld3(g.r_lh + 3 * k), F, Ek);
             }
-        }
+        });
         // LH-DNA (2249-2279)
-        for (int pi = tid; pi < s.n_lnk; pi += nth)
+        for_near<kRecvThreads>(st.ch_l, s.n_lnk, rj, 0.0, tid, nth, s_list, s_wsum, [&](int pi) {
             dhev_recv(s.LH_q[idx] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[idx], rj, ld3(g.r + 3 * s.lnk[pi]), F, E);
+        });
         // LH-tail (2343-2398)
-        for (int t = tid; t < s.n_tail; t += nth)
+        for_near<kRecvThreads>(st.ch_t, s.n_tail, rj, 0.0, tid, nth, s_list, s_wsum, [&](int t) {
             dhev_recv(s.LH_q[idx] * s.t_q[t], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rj, ld3(g.r_t + 3 * t), F, E);
+        });
         // LHG of other cores acting on this bead (2590-2644), energy on the core side
-        for (int x = tid; x < s.n_lhc * s.n_lh_g; x += nth) {
+        for_near<kRecvThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
             int k = x / s.n_lh_g, m = x % s.n_lh_g;
-            if (k == mine) continue;
+            if (k == mine) return;
             dhev_recv(s.LH_q[m] * s.LH_q[idx], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[idx]) / 2, rj,
                       ld3(st.G + 3 * x), F, Ed);
-        }
+        });
     }
     double v[4] = {F.x, F.y, F.z, E};
     block_sum_n<kRecvThreads, 4>(v, red);
@@ -452,29 +569,14 @@ constexpr int kCoreThreads = 320;
 constexpr int kCoreQPer = 2;   // charges per thread: Nq <= 640
 constexpr int kMaxRT = 8;      // redirected tails summed per pass over the LH partners
 
-// Ordered compaction of one candidate per thread: returns the number kept;
-// list[] receives the kept values in thread order.
-__device__ __forceinline__ int compact_ordered(bool keep, int value, int *list, int *wsum) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const unsigned long long m = __ballot(keep);
-    const int before = __popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) wsum[w] = __popcll(m);
-    __syncthreads();
-    int off = 0, tot = 0;
-#pragma unroll
-    for (int i = 0; i < kCoreThreads / 64; i++) {
-        const int c = wsum[i];
-        off += (i < w) ? c : 0;
-        tot += c;
-    }
-    if (keep) list[off + before] = value;
-    __syncthreads();
-    return tot;
-}
-
+// Workgroup (kc, sub) of the gridDim.y workgroups of core kc takes the
+// partners e = sub (mod S) of the charge-owned terms, the other cores
+// k = sub (mod S) of the head-charge terms, and every S-th element of the
+// flat loops; the last workgroup of the core to finish sums the S partial
+// results in order (atomic arrival count, no waiting).
 __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, Frc f, const unsigned char *cc_valid,
-                                                      int nchunk, const double *cc_slots, double *Ecore, double *Ecc,
-                                                      int terms) {
+                                                      int nchunk, const double *cc_slots, double *core_part,
+                                                      int *core_cnt, double *Ecore, double *Ecc, int terms) {
     __shared__ int s_list[kCoreThreads];
     __shared__ double s_pos[3 * kCoreThreads];
     __shared__ int s_wsum[kCoreThreads / 64];
@@ -482,6 +584,8 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
     const int kc = blockIdx.x;
     const int cb = s.core_bead[kc];
     const int tid = threadIdx.x;
+    const int sub = blockIdx.y, S = gridDim.y;
+    const int ft = sub * kCoreThreads + tid, FT = S * kCoreThreads;
     const double3 rc = ld3(g.r + 3 * cb), ac = ld3(g.a + 3 * cb), bc = ld3(g.b + 3 * cb), cc = ld3(g.c + 3 * cb);
     double3 F = zero3(), T = zero3();
     double E = 0.0, Ed = 0.0, Ec = 0.0;
@@ -505,7 +609,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
                     keep = !(sqrt(dot(dc, dc)) > 25.0);
                 }
             }
-            const int cnt = compact_ordered(keep, L, s_list, s_wsum);
+            const int cnt = compact_ordered<kCoreThreads>(keep, L, s_list, s_wsum);
             for (int e = tid; e < cnt; e += kCoreThreads) st3(s_pos + 3 * e, ld3(g.r + 3 * s_list[e]));
             __syncthreads();
 #pragma unroll
@@ -514,7 +618,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
                 if (q < s.Nq) {
                     const double3 Qq = ld3(Qc + 3 * q);
                     const double cq = s.cq[q];
-                    for (int e = 0; e < cnt; e++) {
+                    for (int e = sub; e < cnt; e += S) {
                         const double3 z = Qq - ld3(s_pos + 3 * e);
                         const double d = sqrt(dot(z, z));
                         double g1 = 0.0;
@@ -539,14 +643,14 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
                     keep = !(sqrt(dot(dc, dc)) - s.rmax_g - s.rmax_q > 7.0 + 1e-6);
                 }
             }
-            const int cnt = compact_ordered(keep, k, s_list, s_wsum);
+            const int cnt = compact_ordered<kCoreThreads>(keep, k, s_list, s_wsum);
 #pragma unroll
             for (int i = 0; i < kCoreQPer; i++) {
                 const int q = tid + i * kCoreThreads;
                 if (q < s.Nq) {
                     const double3 Qq = ld3(Qc + 3 * q);
                     const double cq = s.cq[q];
-                    for (int e = 0; e < cnt; e++) {
+                    for (int e = sub; e < cnt; e += S) {
                         const int k2 = s_list[e];
                         for (int m = 0; m < s.n_lh_g; m++)
                             dhev_recv(s.LH_q[m] * cq, s.c4pe, s.debye, s.k_ex, s.LH_hc[m], Qq,
@@ -572,22 +676,27 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
         double3 Gm[kMaxGSites], Fm[kMaxGSites];
         #pragma unroll
         for (int m = 0; m < kMaxGSites; m++) if (m < ng) { Gm[m] = ld3(st.G + 3 * (lho * ng + m)); Fm[m] = zero3(); }
+        // sphere around the head sites: partners beyond it by the cut-off
+        // are left out of the loops below
+        double3 hc;
+        double hr;
+        points_sphere(Gm, ng, hc, hr);
         // LHG-tail (2403-2476)
-        for (int t = tid; t < s.n_tail; t += blockDim.x) {
+        for_near<kCoreThreads>(st.ch_t, s.n_tail, hc, hr, ft, FT, s_list, s_wsum, [&](int t) {
             double3 rt = ld3(g.r_t + 3 * t);
 #pragma unroll
             for (int m = 0; m < kMaxGSites; m++) if (m < ng)
                 dhev_recv(s.LH_q[m] * s.t_q[t], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], Gm[m], rt, Fm[m], E);
-        }
+        });
         // LHG-DNA (2481-2522)
-        for (int pi = tid; pi < s.n_lnk; pi += blockDim.x) {
+        for_near<kCoreThreads>(st.ch_l, s.n_lnk, hc, hr, ft, FT, s_list, s_wsum, [&](int pi) {
             double3 rl = ld3(g.r + 3 * s.lnk[pi]);
 #pragma unroll
             for (int m = 0; m < kMaxGSites; m++) if (m < ng)
                 dhev_recv(s.LH_q[m] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[m], Gm[m], rl, Fm[m], E);
-        }
+        });
         // LHG-charges of every other core (2526-2586), culled per core
-        for (int k = 0; k < s.n_c; k++) {
+        for (int k = sub; k < s.n_c; k += S) {
             int ob = s.core_bead[k];
             if (ob == cb) continue;
             double3 dc = ld3(g.r + 3 * ob) - rc;
@@ -601,27 +710,27 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
             }
         }
         // LHG-free LH beads of other cores (2590-2644)
-        for (int jl = tid; jl < s.n_LH; jl += blockDim.x) {
-            if (jl / s.nlh == lho) continue;
+        for_near<kCoreThreads>(st.ch_lh, s.n_LH, hc, hr, ft, FT, s_list, s_wsum, [&](int jl) {
+            if (jl / s.nlh == lho) return;
             int i2 = jl % s.nlh;
             i2 = (i2 < s.n_lh_n) ? i2 : i2 + s.n_lh_g;
             double3 rl = ld3(g.r_lh + 3 * jl);
 #pragma unroll
             for (int m = 0; m < kMaxGSites; m++) if (m < ng)
                 dhev_recv(s.LH_q[m] * s.LH_q[i2], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[i2]) / 2, Gm[m], rl, Fm[m], E);
-        }
+        });
         // LHG-LHG with the other LH-bound cores (2649-2719)
-        for (int x = tid; x < s.n_lhc * ng; x += blockDim.x) {
+        for_near<kCoreThreads>(st.ch_g, s.n_lhc * ng, hc, hr, ft, FT, s_list, s_wsum, [&](int x) {
             int k = x / ng, m1 = x % ng;
-            if (k == lho) continue;
+            if (k == lho) return;
             double3 G2 = ld3(st.G + 3 * x);
 #pragma unroll
             for (int m = 0; m < kMaxGSites; m++) if (m < ng)
                 dhev_recv(s.LH_q[m] * s.LH_q[m1], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[m1]) / 2, Gm[m], G2, Fm[m],
                           (k > lho) ? E : Ed);
-        }
+        });
         // head - first C-term bead spring (2727-2777): force x10, torque x1
-        if (tid == 0 && s.n_LH > 0) {
+        if (ft == 0 && s.n_LH > 0) {
             double3 p = ld3(g.r_lh + 3 * (size_t)lho * s.nlh);
             #pragma unroll
         for (int m = 0; m < kMaxGSites; m++) if (m < ng) {
@@ -657,21 +766,24 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
                 tq[e] = s.t_q[tj];
                 Ft[e] = zero3();
             }
-            for (int i = tid; i < s.n_LH; i += blockDim.x) {
+            double3 tc;
+            double tr;
+            points_sphere(rt, ne, tc, tr);
+            for_near<kCoreThreads>(st.ch_lh, s.n_LH, tc, tr, ft, FT, s_list, s_wsum, [&](int i) {
                 int idx = i % s.nlh;
                 idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
                 const double3 rl = ld3(g.r_lh + 3 * i);
 #pragma unroll
                 for (int e = 0; e < kMaxRT; e++)
                     if (e < ne) dhev_recv(s.LH_q[idx] * tq[e], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rt[e], rl, Ft[e], Ed);
-            }
-            for (int x = tid; x < s.n_lhc * s.n_lh_g; x += blockDim.x) {
+            });
+            for_near<kCoreThreads>(st.ch_g, s.n_lhc * s.n_lh_g, tc, tr, ft, FT, s_list, s_wsum, [&](int x) {
                 const int m = x % s.n_lh_g;
                 const double3 Gx = ld3(st.G + 3 * x);
 #pragma unroll
                 for (int e = 0; e < kMaxRT; e++)
                     if (e < ne) dhev_recv(s.LH_q[m] * tq[e], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], rt[e], Gx, Ft[e], Ed);
-            }
+            });
             double v[3 * kMaxRT];
 #pragma unroll
             for (int e = 0; e < kMaxRT; e++) {
@@ -691,7 +803,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
         }
     }
     // --- anchor springs of this core's own tails (1971-2022); energy on the tail side ---
-    if ((terms & TERM_TAIL) && tid == 0 && s.n_tail > 0) {
+    if ((terms & TERM_TAIL) && ft == 0 && s.n_tail > 0) {
         for (int jj = 0; jj < s.n_t; jj++) {
             int tj = kc * s.n_t + jj;
             if (s.t_fix[tj] != 1) continue;
@@ -706,7 +818,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
     //     of the slot matrix (func.cpp:1489-1566) ---
     if (terms & TERM_ELEC) {
         double3 Tw = zero3();
-        for (int x = tid; x < s.n_c * nchunk; x += blockDim.x) {
+        for (int x = ft; x < s.n_c * nchunk; x += FT) {
             const int o = x / nchunk, ch = x % nchunk;
             if (o == kc) continue;
             const bool first = kc < o;
@@ -724,15 +836,27 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
         }
         T += make_double3(dot(ac, Tw), dot(bc, Tw), dot(cc, Tw));
     }
-    // block reduction in fixed order
+    // block reduction in fixed order, then the core's S partial sums in order
     double v[8] = {F.x, F.y, F.z, T.x, T.y, T.z, E, Ec};
     block_sum_n<kCoreThreads, 8>(v, red);
     if (tid == 0) {
-        double *Fc = f.F + 3 * cb, *Tc = f.T + 3 * cb;
-        Fc[0] += v[0]; Fc[1] += v[1]; Fc[2] += v[2];
-        Tc[0] += v[3]; Tc[1] += v[4]; Tc[2] += v[5];
-        if (Ecore) Ecore[kc] = v[6];
-        if (Ecc) Ecc[kc] = v[7];
+        double *part = core_part + (size_t)kc * S * 8;
+#pragma unroll
+        for (int i = 0; i < 8; i++) part[sub * 8 + i] = v[i];
+        __threadfence();
+        if (atomicAdd(core_cnt + kc, 1) == S - 1) {
+            __threadfence();
+            double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int j = 0; j < S; j++)
+#pragma unroll
+                for (int i = 0; i < 8; i++) t[i] += part[j * 8 + i];
+            core_cnt[kc] = 0;
+            double *Fc = f.F + 3 * cb, *Tc = f.T + 3 * cb;
+            Fc[0] += t[0]; Fc[1] += t[1]; Fc[2] += t[2];
+            Tc[0] += t[3]; Tc[1] += t[4]; Tc[2] += t[5];
+            if (Ecore) Ecore[kc] = t[6];
+            if (Ecc) Ecc[kc] = t[7];
+        }
     }
 }
 
@@ -799,8 +923,10 @@ __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, Sites st,
         for (int k = threadIdx.x; k < s.Nq; k += blockDim.x) {
             const double3 Qk = ld3(st.Q + 3 * ((size_t)k1 * s.Nq + k));
             const double keq = s.k_e * s.cq[k];
-            double3 Fk = zero3();
-            for (int l = 0; l < l1 - l0; l++) {
+            // Fk = sum_l f_l and Gk = sum_l g_l (Q_l - r2), f_l = g_l (Qk - Q_l)
+            double3 Fk = zero3(), Gk = zero3();
+            const int nl = l1 - l0;
+            for (int l = 0; l < nl; l++) {
                 const double *d = sq + 7 * l;
                 const double3 z = make_double3(Qk.x - d[0], Qk.y - d[1], Qk.z - d[2]);
                 // Debye-Hueckel + excluded volume of the reference (dh_core, ev_core)
@@ -808,7 +934,7 @@ __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, Sites st,
                 const double r2s = dot(z, z);
                 const double inv = rsqrt_f64(r2s);
                 const double dist = r2s * inv;
-                const double t = keq * d[3] * exp(-kappa * dist);
+                const double t = keq * d[3] * exp_neg(-kappa * dist);
                 if (WITH_E) E += t * inv;
                 double g1d = t * (kappa * dist + 1.0) * (inv * inv * inv);
                 if (dist <= 8.0) {
@@ -816,15 +942,19 @@ __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, Sites st,
                     if (WITH_E) E += kex * (x6 * x6 - x6);
                     g1d += kex * (c12 * x13 - c6 * x7) * inv;
                 }
-                const double3 fk = g1d * z;
-                Fk += fk;
-                // torque on c2 about its centre from -fk at lever (Q_l - r2)
-                T2.x -= d[5] * fk.z - d[6] * fk.y;
-                T2.y -= d[6] * fk.x - d[4] * fk.z;
-                T2.z -= d[4] * fk.y - d[5] * fk.x;
+                Fk.x = __builtin_fma(g1d, z.x, Fk.x);
+                Fk.y = __builtin_fma(g1d, z.y, Fk.y);
+                Fk.z = __builtin_fma(g1d, z.z, Fk.z);
+                Gk.x = __builtin_fma(g1d, d[4], Gk.x);
+                Gk.y = __builtin_fma(g1d, d[5], Gk.y);
+                Gk.z = __builtin_fma(g1d, d[6], Gk.z);
             }
             F += Fk;
             T1 += torque_rel(Fk, ld3(s.cpos + 3 * k), a1, bb1, c1);
+            // torque on c2 about its centre from -f_l at lever (Q_l - r2):
+            // sum_l (Q_l - r2) x (-g_l (Qk - Q_l)) = -Gk x (Qk - r2), since
+            // (Q_l - r2) x (Q_l - r2) = 0
+            T2 -= cross(Gk, Qk - r2);
         }
         double v[10] = {F.x, F.y, F.z, T1.x, T1.y, T1.z, T2.x, T2.y, T2.z, E};
         block_sum_n<kCCThreads, 10>(v, red);
@@ -860,6 +990,8 @@ static inline int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
 void launch_sites(const Stat &s, const Geo &g, const Sites &st, hipStream_t q) {
     long long tot = (long long)s.n_c * s.Nq + (long long)s.n_lhc * s.n_lh_g;
     if (tot > 0) k_sites<<<nblk(tot, 256), 256, 0, q>>>(s, g, st);
+    const int nch = nchunks(s.n_tail) + nchunks(s.n_LH) + nchunks(s.n_lhc * s.n_lh_g) + nchunks(s.n_lnk);
+    if (nch > 0) k_chunks<<<nblk(nch, 4), 256, 0, q>>>(s, g, st);
 }
 
 void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, const ForceScratch &w,
@@ -904,7 +1036,8 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
         }
     }
     M(K_CORE, 0);
-    k_core<<<s.n_c, kCoreThreads, 0, q>>>(s, g, st, f, w.cc_valid, w.cc_nchunk, w.cc_slots, E(off_core), E(off_cc), terms);
+    k_core<<<dim3(s.n_c, w.core_split), kCoreThreads, 0, q>>>(s, g, st, f, w.cc_valid, w.cc_nchunk, w.cc_slots,
+                                                            w.core_part, w.core_cnt, E(off_core), E(off_cc), terms);
     M(K_CORE, 1);
 }
 

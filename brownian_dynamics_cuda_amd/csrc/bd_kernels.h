@@ -12,6 +12,11 @@ struct ForceScratch {
     double *cc_slots;         // n_c * n_c * nchunk * 10
     int cc_nchunk;
     int cc_grid;
+    // k_core runs core_split workgroups per core; each leaves 8 partial sums
+    // in core_part and the last to finish (core_cnt) adds them up in order
+    int core_split;
+    double *core_part;        // n_c * core_split * 8
+    int *core_cnt;            // n_c, zero between passes
 };
 
 // forces (bd_forces.hip)
@@ -57,7 +62,10 @@ struct CholWork {
 };
 void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, const CholWork &c,
                 hipStream_t q);
-void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark = nullptr, void *mark_arg = nullptr);
+// gate (optional) is recorded on q after the launch of block column gate_kb
+// (after the last one when gate_kb is past the end)
+void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark = nullptr, void *mark_arg = nullptr,
+                     hipEvent_t gate = nullptr, int gate_kb = -1);
 void launch_noise(const Stat &s, int n_hi, int n_D3, const CholWork &c, const double *p,
                   const double *rad_all, double *rd, hipStream_t q);
 
