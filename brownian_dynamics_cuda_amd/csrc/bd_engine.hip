@@ -56,7 +56,7 @@ int fail(int code, const char *fmt, ...) {
 const char *kKernelNames[K_COUNT] = {
     "sites", "mech", "linker", "tail", "lh", "core", "cc_list", "cc_work", "rpy",
     "potrf", "panel_copy", "chol_step", "noise", "frames", "betas", "rotate", "translate", "average",
-    "commit", "energy_reduce"};
+    "commit", "energy_reduce", "cc_gather"};
 
 }  // namespace
 

@@ -574,9 +574,8 @@ constexpr int kMaxRT = 8;      // redirected tails summed per pass over the LH p
 // k = sub (mod S) of the head-charge terms, and every S-th element of the
 // flat loops; the last workgroup of the core to finish sums the S partial
 // results in order (atomic arrival count, no waiting).
-__global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, Frc f, const unsigned char *cc_valid,
-                                                      int nchunk, const double *cc_slots, double *core_part,
-                                                      int *core_cnt, double *Ecore, double *Ecc, int terms) {
+__global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, Frc f, double *core_part,
+                                                      int *core_cnt, double *Ecore, int terms) {
     __shared__ int s_list[kCoreThreads];
     __shared__ double s_pos[3 * kCoreThreads];
     __shared__ int s_wsum[kCoreThreads / 64];
@@ -588,7 +587,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
     const int ft = sub * kCoreThreads + tid, FT = S * kCoreThreads;
     const double3 rc = ld3(g.r + 3 * cb), ac = ld3(g.a + 3 * cb), bc = ld3(g.b + 3 * cb), cc = ld3(g.c + 3 * cb);
     double3 F = zero3(), T = zero3();
-    double E = 0.0, Ed = 0.0, Ec = 0.0;
+    double E = 0.0, Ed = 0.0, Ec = 0.0;  // Ec: kept for the layout of the partial sums
     const double *Qc = st.Q + 3 * (size_t)kc * s.Nq;
     const int lho = s.lh_of_core[kc];
 
@@ -814,28 +813,6 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
             T += torque_rel(hs, ld3(s.tail_pos + 3 * jj), ac, bc, cc);
         }
     }
-    // --- core-core charge sums: this core's row (as c1) and column (as c2)
-    //     of the slot matrix (func.cpp:1489-1566) ---
-    if (terms & TERM_ELEC) {
-        double3 Tw = zero3();
-        for (int x = ft; x < s.n_c * nchunk; x += FT) {
-            const int o = x / nchunk, ch = x % nchunk;
-            if (o == kc) continue;
-            const bool first = kc < o;
-            const size_t pidx = first ? (size_t)kc * s.n_c + o : (size_t)o * s.n_c + kc;
-            if (!cc_valid[pidx]) continue;
-            const double *sl = cc_slots + 10 * (pidx * nchunk + ch);
-            if (first) {
-                F += make_double3(sl[0], sl[1], sl[2]);
-                T += make_double3(sl[3], sl[4], sl[5]);
-                Ec += sl[9];
-            } else {
-                F -= make_double3(sl[0], sl[1], sl[2]);
-                Tw += make_double3(sl[6], sl[7], sl[8]);
-            }
-        }
-        T += make_double3(dot(ac, Tw), dot(bc, Tw), dot(cc, Tw));
-    }
     // block reduction in fixed order, then the core's S partial sums in order
     double v[8] = {F.x, F.y, F.z, T.x, T.y, T.z, E, Ec};
     block_sum_n<kCoreThreads, 8>(v, red);
@@ -855,8 +832,52 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
             Fc[0] += t[0]; Fc[1] += t[1]; Fc[2] += t[2];
             Tc[0] += t[3]; Tc[1] += t[4]; Tc[2] += t[5];
             if (Ecore) Ecore[kc] = t[6];
-            if (Ecc) Ecc[kc] = t[7];
         }
+    }
+}
+
+// The core-core charge sums of core kc: its row (as c1) and column (as c2)
+// of the slot matrix that k_cc_work fills (func.cpp:1489-1566), one
+// wavefront per core, lanes over the (other core, chunk) slots in index
+// order and a fixed shuffle tree; adds to F, T of the core bead and owns
+// the energy of the pairs where kc is c1.
+__global__ void __launch_bounds__(64) k_cc_gather(Stat s, Geo g, Frc f, const unsigned char *cc_valid, int nchunk,
+                                                   const double *cc_slots, double *Ecc, int terms) {
+    const int kc = blockIdx.x, lane = threadIdx.x;
+    const int cb = s.core_bead[kc];
+    double3 F = zero3(), T = zero3(), Tw = zero3();
+    double Ec = 0.0;
+    // without the electrostatic terms only the energy slot is cleared
+    const int nx = (terms & TERM_ELEC) ? s.n_c * nchunk : 0;
+    for (int x = lane; x < nx; x += 64) {
+        const int o = x / nchunk, ch = x % nchunk;
+        if (o == kc) continue;
+        const bool first = kc < o;
+        const size_t pidx = first ? (size_t)kc * s.n_c + o : (size_t)o * s.n_c + kc;
+        if (!cc_valid[pidx]) continue;
+        const double *sl = cc_slots + 10 * (pidx * nchunk + ch);
+        if (first) {
+            F += make_double3(sl[0], sl[1], sl[2]);
+            T += make_double3(sl[3], sl[4], sl[5]);
+            Ec += sl[9];
+        } else {
+            F -= make_double3(sl[0], sl[1], sl[2]);
+            Tw += make_double3(sl[6], sl[7], sl[8]);
+        }
+    }
+    double v[10] = {F.x, F.y, F.z, T.x, T.y, T.z, Tw.x, Tw.y, Tw.z, Ec};
+#pragma unroll
+    for (int q = 0; q < 10; q++)
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
+    if (lane == 0) {
+        const double3 ac = ld3(g.a + 3 * cb), bc = ld3(g.b + 3 * cb), cc = ld3(g.c + 3 * cb);
+        const double3 tw = make_double3(v[6], v[7], v[8]);
+        double *Fc = f.F + 3 * cb, *Tc = f.T + 3 * cb;
+        Fc[0] += v[0]; Fc[1] += v[1]; Fc[2] += v[2];
+        Tc[0] += v[3] + dot(ac, tw);
+        Tc[1] += v[4] + dot(bc, tw);
+        Tc[2] += v[5] + dot(cc, tw);
+        if (Ecc) Ecc[kc] = v[9];
     }
 }
 
@@ -1001,9 +1022,9 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
     auto E = [&](int off) { return Epart ? Epart + off : nullptr; };
     auto M = [&](int id, int ph) { if (mark) mark(arg, id, ph); };
     M(K_SITES, 0); launch_sites(s, g, st, q); M(K_SITES, 1);
-    // k_mech assigns F, T of every main bead and k_linker then adds to the
-    // linker beads; k_tail and k_lh assign their own arrays; k_core adds to
-    // the cores after k_mech and gathers k_cc_work's slots
+    // k_mech assigns F, T of every main bead, then k_linker adds to the
+    // linker beads and k_core to the cores; k_tail and k_lh assign their own
+    // arrays; k_cc_gather adds k_cc_work's slots to the cores after the join
     hipStream_t qt = q, ql = q, qc = q;
     if (fs) {
         (void)hipEventRecord(fs->fork, q);
@@ -1029,16 +1050,19 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
     }
     M(K_MECH, 0); k_mech<<<nblk(s.n, 64), 64, 0, q>>>(s, g, f, E(off_mech), terms); M(K_MECH, 1);
     if (s.n_lnk) { M(K_LINKER, 0); k_linker<<<s.n_lnk, kRecvThreads, 0, q>>>(s, g, st, f, E(off_lnk), terms); M(K_LINKER, 1); }
+    M(K_CORE, 0);
+    k_core<<<dim3(s.n_c, w.core_split), kCoreThreads, 0, q>>>(s, g, st, f, w.core_part, w.core_cnt, E(off_core),
+                                                            terms);
+    M(K_CORE, 1);
     if (fs) {
         for (int i = 0; i < 3; i++) {
             (void)hipEventRecord(fs->join[i], fs->b[i]);
             (void)hipStreamWaitEvent(q, fs->join[i], 0);
         }
     }
-    M(K_CORE, 0);
-    k_core<<<dim3(s.n_c, w.core_split), kCoreThreads, 0, q>>>(s, g, st, f, w.cc_valid, w.cc_nchunk, w.cc_slots,
-                                                            w.core_part, w.core_cnt, E(off_core), E(off_cc), terms);
-    M(K_CORE, 1);
+    M(K_CCGATHER, 0);
+    k_cc_gather<<<s.n_c, 64, 0, q>>>(s, g, f, w.cc_valid, w.cc_nchunk, w.cc_slots, E(off_cc), terms);
+    M(K_CCGATHER, 1);
 }
 
 int energy_slots(const Stat &s) { return s.n + s.n_lnk + s.n_tail + s.n_LH + 2 * s.n_c; }
