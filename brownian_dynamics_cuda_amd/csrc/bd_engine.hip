@@ -356,7 +356,8 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
         c->fstr[0].b[i] = c->fstr[1].b[i] = b;
     }
     for (auto &fs : c->fstr) {
-        bool ok = hipEventCreateWithFlags(&fs.fork, hipEventDisableTiming) == hipSuccess;
+        bool ok = hipEventCreateWithFlags(&fs.fork, hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&fs.mech, hipEventDisableTiming) == hipSuccess;
         for (auto &j : fs.join) ok = ok && hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess;
         if (!ok) return bail(fail(BD_E_HIP, "hipEventCreate failed"));
     }
@@ -482,8 +483,11 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
         r |= c->alloc(&c->fw.cc_list, 2 * (size_t)std::max(npairs, 1LL));
         r |= c->alloc(&c->fw.cc_count, 1);
         r |= c->alloc(&c->fw.cc_slots, (size_t)n_c * n_c * c->fw.cc_nchunk * 10);
-        c->fw.core_split = 4;
-        r |= c->alloc(&c->fw.core_part, (size_t)n_c * c->fw.core_split * 8);
+        c->fw.core_split[0] = 1;
+        c->fw.core_split[1] = 2;
+        c->fw.core_split[2] = 1;
+        r |= c->alloc(&c->fw.core_part,
+                      (size_t)n_c * (c->fw.core_split[0] + c->fw.core_split[1] + c->fw.core_split[2]) * 8);
         r |= c->alloc(&c->fw.core_cnt, n_c);
     }
     // Cholesky workspace
@@ -546,6 +550,7 @@ int bd_destroy(bd_context *c) {
         }
     for (auto &fs : c->fstr) {
         if (fs.fork) (void)hipEventDestroy(fs.fork);
+        if (fs.mech) (void)hipEventDestroy(fs.mech);
         for (auto &j : fs.join)
             if (j) (void)hipEventDestroy(j);
     }

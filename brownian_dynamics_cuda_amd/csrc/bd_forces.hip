@@ -569,13 +569,16 @@ constexpr int kCoreThreads = 320;
 constexpr int kCoreQPer = 2;   // charges per thread: Nq <= 640
 constexpr int kMaxRT = 8;      // redirected tails summed per pass over the LH partners
 
-// Workgroup (kc, sub) of the gridDim.y workgroups of core kc takes the
-// partners e = sub (mod S) of the charge-owned terms, the other cores
-// k = sub (mod S) of the head-charge terms, and every S-th element of the
-// flat loops; the last workgroup of the core to finish sums the S partial
-// results in order (atomic arrival count, no waiting).
+// The gridDim.y = S0 + S1 + S2 workgroups of core kc take three phases:
+// y < S0 the charge-owned terms, then S1 workgroups the core's own
+// globular-head sites, then S2 the redirected tails and anchor springs.
+// Workgroup sub of the S of its phase takes the partners e = sub (mod S)
+// of the charge-owned terms, the other cores k = sub (mod S) of the
+// head-charge terms, and every S-th element of the flat loops.  The last
+// workgroup of the core to finish sums the partial results in order of y
+// (atomic arrival count, no waiting).
 __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, Frc f, double *core_part,
-                                                      int *core_cnt, double *Ecore, int terms) {
+                                                      int *core_cnt, double *Ecore, int terms, int S0, int S1) {
     __shared__ int s_list[kCoreThreads];
     __shared__ double s_pos[3 * kCoreThreads];
     __shared__ int s_wsum[kCoreThreads / 64];
@@ -583,7 +586,10 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
     const int kc = blockIdx.x;
     const int cb = s.core_bead[kc];
     const int tid = threadIdx.x;
-    const int sub = blockIdx.y, S = gridDim.y;
+    const int NS = gridDim.y, y = blockIdx.y;
+    const int phase = y < S0 ? 0 : (y < S0 + S1 ? 1 : 2);
+    const int S = phase == 0 ? S0 : (phase == 1 ? S1 : NS - S0 - S1);
+    const int sub = phase == 0 ? y : (phase == 1 ? y - S0 : y - S0 - S1);
     const int ft = sub * kCoreThreads + tid, FT = S * kCoreThreads;
     const double3 rc = ld3(g.r + 3 * cb), ac = ld3(g.a + 3 * cb), bc = ld3(g.b + 3 * cb), cc = ld3(g.c + 3 * cb);
     double3 F = zero3(), T = zero3();
@@ -592,85 +598,87 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
     const int lho = s.lh_of_core[kc];
 
     // --- charge-owned terms: thread per charge q of this core ---
-    double3 Fq[kCoreQPer];
+    if (phase == 0) {
+        double3 Fq[kCoreQPer];
 #pragma unroll
-    for (int i = 0; i < kCoreQPer; i++) Fq[i] = zero3();
-    if ((terms & TERM_ELEC) && s.elig[cb]) {
-        // core-linker (func.cpp:1356-1487): force on the charge
-        for (int base = 0; base < s.n_lnk; base += kCoreThreads) {
-            const int pi = base + tid;
-            bool keep = false;
-            int L = 0;
-            if (pi < s.n_lnk) {
-                L = s.lnk[pi];
-                if (s.elig[L]) {
-                    const double3 dc = rc - ld3(g.r + 3 * L);
-                    keep = !(sqrt(dot(dc, dc)) > 25.0);
-                }
-            }
-            const int cnt = compact_ordered<kCoreThreads>(keep, L, s_list, s_wsum);
-            for (int e = tid; e < cnt; e += kCoreThreads) st3(s_pos + 3 * e, ld3(g.r + 3 * s_list[e]));
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < kCoreQPer; i++) {
-                const int q = tid + i * kCoreThreads;
-                if (q < s.Nq) {
-                    const double3 Qq = ld3(Qc + 3 * q);
-                    const double cq = s.cq[q];
-                    for (int e = sub; e < cnt; e += S) {
-                        const double3 z = Qq - ld3(s_pos + 3 * e);
-                        const double d = sqrt(dot(z, z));
-                        double g1 = 0.0;
-                        if (abs(cb - s_list[e]) > 1) g1 += dh_core(s.k_e * s.q_l * cq, s.debye, d, E);
-                        if (d <= 8.0 && cq > 0) g1 += ev_core(s.k_ex, s.sig_dc, d, E);
-                        Fq[i] += (g1 / d) * z;
+        for (int i = 0; i < kCoreQPer; i++) Fq[i] = zero3();
+        if ((terms & TERM_ELEC) && s.elig[cb]) {
+            // core-linker (func.cpp:1356-1487): force on the charge
+            for (int base = 0; base < s.n_lnk; base += kCoreThreads) {
+                const int pi = base + tid;
+                bool keep = false;
+                int L = 0;
+                if (pi < s.n_lnk) {
+                    L = s.lnk[pi];
+                    if (s.elig[L]) {
+                        const double3 dc = rc - ld3(g.r + 3 * L);
+                        keep = !(sqrt(dot(dc, dc)) > 25.0);
                     }
                 }
-            }
-            __syncthreads();
-        }
-    }
-    if (terms & TERM_LH) {
-        // LHG of other cores acting on this charge (func.cpp:2526-2586, charge side)
-        for (int base = 0; base < s.n_lhc; base += kCoreThreads) {
-            const int k = base + tid;
-            bool keep = false;
-            if (k < s.n_lhc) {
-                const int ob = s.lhc_bead[k];
-                if (ob != cb) {
-                    const double3 dc = ld3(g.r + 3 * ob) - rc;
-                    keep = !(sqrt(dot(dc, dc)) - s.rmax_g - s.rmax_q > 7.0 + 1e-6);
-                }
-            }
-            const int cnt = compact_ordered<kCoreThreads>(keep, k, s_list, s_wsum);
+                const int cnt = compact_ordered<kCoreThreads>(keep, L, s_list, s_wsum);
+                for (int e = tid; e < cnt; e += kCoreThreads) st3(s_pos + 3 * e, ld3(g.r + 3 * s_list[e]));
+                __syncthreads();
 #pragma unroll
-            for (int i = 0; i < kCoreQPer; i++) {
-                const int q = tid + i * kCoreThreads;
-                if (q < s.Nq) {
-                    const double3 Qq = ld3(Qc + 3 * q);
-                    const double cq = s.cq[q];
-                    for (int e = sub; e < cnt; e += S) {
-                        const int k2 = s_list[e];
-                        for (int m = 0; m < s.n_lh_g; m++)
-                            dhev_recv(s.LH_q[m] * cq, s.c4pe, s.debye, s.k_ex, s.LH_hc[m], Qq,
-                                      ld3(st.G + 3 * (k2 * s.n_lh_g + m)), Fq[i], Ed);
+                for (int i = 0; i < kCoreQPer; i++) {
+                    const int q = tid + i * kCoreThreads;
+                    if (q < s.Nq) {
+                        const double3 Qq = ld3(Qc + 3 * q);
+                        const double cq = s.cq[q];
+                        for (int e = sub; e < cnt; e += S) {
+                            const double3 z = Qq - ld3(s_pos + 3 * e);
+                            const double d = sqrt(dot(z, z));
+                            double g1 = 0.0;
+                            if (abs(cb - s_list[e]) > 1) g1 += dh_core(s.k_e * s.q_l * cq, s.debye, d, E);
+                            if (d <= 8.0 && cq > 0) g1 += ev_core(s.k_ex, s.sig_dc, d, E);
+                            Fq[i] += (g1 / d) * z;
+                        }
                     }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
-    }
+        if (terms & TERM_LH) {
+            // LHG of other cores acting on this charge (func.cpp:2526-2586, charge side)
+            for (int base = 0; base < s.n_lhc; base += kCoreThreads) {
+                const int k = base + tid;
+                bool keep = false;
+                if (k < s.n_lhc) {
+                    const int ob = s.lhc_bead[k];
+                    if (ob != cb) {
+                        const double3 dc = ld3(g.r + 3 * ob) - rc;
+                        keep = !(sqrt(dot(dc, dc)) - s.rmax_g - s.rmax_q > 7.0 + 1e-6);
+                    }
+                }
+                const int cnt = compact_ordered<kCoreThreads>(keep, k, s_list, s_wsum);
 #pragma unroll
-    for (int i = 0; i < kCoreQPer; i++) {
-        const int q = tid + i * kCoreThreads;
-        if (q < s.Nq) {
-            F += Fq[i];
-            T += torque_rel(Fq[i], ld3(s.cpos + 3 * q), ac, bc, cc);
+                for (int i = 0; i < kCoreQPer; i++) {
+                    const int q = tid + i * kCoreThreads;
+                    if (q < s.Nq) {
+                        const double3 Qq = ld3(Qc + 3 * q);
+                        const double cq = s.cq[q];
+                        for (int e = sub; e < cnt; e += S) {
+                            const int k2 = s_list[e];
+                            for (int m = 0; m < s.n_lh_g; m++)
+                                dhev_recv(s.LH_q[m] * cq, s.c4pe, s.debye, s.k_ex, s.LH_hc[m], Qq,
+                                          ld3(st.G + 3 * (k2 * s.n_lh_g + m)), Fq[i], Ed);
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kCoreQPer; i++) {
+            const int q = tid + i * kCoreThreads;
+            if (q < s.Nq) {
+                F += Fq[i];
+                T += torque_rel(Fq[i], ld3(s.cpos + 3 * q), ac, bc, cc);
+            }
         }
     }
 
     // --- own globular-head sites (LH-bound cores) ---
-    if ((terms & TERM_LH) && lho >= 0) {
+    if (phase == 1 && (terms & TERM_LH) && lho >= 0) {
         const int ng = s.n_lh_g;
         double3 Gm[kMaxGSites], Fm[kMaxGSites];
         #pragma unroll
@@ -694,19 +702,30 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
             for (int m = 0; m < kMaxGSites; m++) if (m < ng)
                 dhev_recv(s.LH_q[m] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[m], Gm[m], rl, Fm[m], E);
         });
-        // LHG-charges of every other core (2526-2586), culled per core
-        for (int k = sub; k < s.n_c; k += S) {
-            int ob = s.core_bead[k];
-            if (ob == cb) continue;
-            double3 dc = ld3(g.r + 3 * ob) - rc;
-            if (sqrt(dot(dc, dc)) - s.rmax_g - s.rmax_q > 7.0 + 1e-6) continue;
-            const double *Qk = st.Q + 3 * (size_t)k * s.Nq;
-            for (int q = tid; q < s.Nq; q += blockDim.x) {
-                double3 Qq = ld3(Qk + 3 * q);
-#pragma unroll
-                for (int m = 0; m < kMaxGSites; m++) if (m < ng)
-                    dhev_recv(s.LH_q[m] * s.cq[q], s.c4pe, s.debye, s.k_ex, s.LH_hc[m], Gm[m], Qq, Fm[m], E);
+        // LHG-charges of every other core (2526-2586): the cores k = sub
+        // (mod S) whose charges (within rmax_q of the core) can come within
+        // reach of the head sphere, listed by the workgroup in index order
+        for (int base = 0; base < s.n_c; base += kCoreThreads) {
+            const int k = base + tid;
+            bool keep = false;
+            if (k < s.n_c && k % S == sub) {
+                const int ob = s.core_bead[k];
+                if (ob != cb) {
+                    const double3 dc = ld3(g.r + 3 * ob) - hc;
+                    keep = !(sqrt(dot(dc, dc)) - hr - s.rmax_q > kReach);
+                }
             }
+            const int cnt = compact_ordered<kCoreThreads>(keep, k, s_list, s_wsum);
+            for (int e = 0; e < cnt; e++) {
+                const double *Qk = st.Q + 3 * (size_t)s_list[e] * s.Nq;
+                for (int q = tid; q < s.Nq; q += blockDim.x) {
+                    const double3 Qq = ld3(Qk + 3 * q);
+#pragma unroll
+                    for (int m = 0; m < kMaxGSites; m++) if (m < ng)
+                        dhev_recv(s.LH_q[m] * s.cq[q], s.c4pe, s.debye, s.k_ex, s.LH_hc[m], Gm[m], Qq, Fm[m], E);
+                }
+            }
+            __syncthreads();
         }
         // LHG-free LH beads of other cores (2590-2644)
         for_near<kCoreThreads>(st.ch_lh, s.n_LH, hc, hr, ft, FT, s_list, s_wsum, [&](int jl) {
@@ -753,7 +772,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
 
     // --- anchored tails whose LH forces are redirected to this core, up to
     //     kMaxRT per pass over the LH partners ---
-    if (terms & TERM_LH) {
+    if (phase == 2 && (terms & TERM_LH)) {
         for (int e0 = s.rt_off[kc]; e0 < s.rt_off[kc + 1]; e0 += kMaxRT) {
             const int ne = min(kMaxRT, s.rt_off[kc + 1] - e0);
             double3 rt[kMaxRT], Ft[kMaxRT];
@@ -802,7 +821,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
         }
     }
     // --- anchor springs of this core's own tails (1971-2022); energy on the tail side ---
-    if ((terms & TERM_TAIL) && ft == 0 && s.n_tail > 0) {
+    if (phase == 2 && (terms & TERM_TAIL) && ft == 0 && s.n_tail > 0) {
         for (int jj = 0; jj < s.n_t; jj++) {
             int tj = kc * s.n_t + jj;
             if (s.t_fix[tj] != 1) continue;
@@ -813,18 +832,18 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
             T += torque_rel(hs, ld3(s.tail_pos + 3 * jj), ac, bc, cc);
         }
     }
-    // block reduction in fixed order, then the core's S partial sums in order
+    // block reduction in fixed order, then the core's 3 S partial sums in order
     double v[8] = {F.x, F.y, F.z, T.x, T.y, T.z, E, Ec};
     block_sum_n<kCoreThreads, 8>(v, red);
     if (tid == 0) {
-        double *part = core_part + (size_t)kc * S * 8;
+        double *part = core_part + (size_t)kc * NS * 8;
 #pragma unroll
-        for (int i = 0; i < 8; i++) part[sub * 8 + i] = v[i];
+        for (int i = 0; i < 8; i++) part[blockIdx.y * 8 + i] = v[i];
         __threadfence();
-        if (atomicAdd(core_cnt + kc, 1) == S - 1) {
+        if (atomicAdd(core_cnt + kc, 1) == NS - 1) {
             __threadfence();
             double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            for (int j = 0; j < S; j++)
+            for (int j = 0; j < NS; j++)
 #pragma unroll
                 for (int i = 0; i < 8; i++) t[i] += part[j * 8 + i];
             core_cnt[kc] = 0;
@@ -1023,8 +1042,9 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
     auto M = [&](int id, int ph) { if (mark) mark(arg, id, ph); };
     M(K_SITES, 0); launch_sites(s, g, st, q); M(K_SITES, 1);
     // k_mech assigns F, T of every main bead, then k_linker adds to the
-    // linker beads and k_core to the cores; k_tail and k_lh assign their own
-    // arrays; k_cc_gather adds k_cc_work's slots to the cores after the join
+    // linker beads and k_core to the cores (disjoint beads, so the two run
+    // side by side); k_tail and k_lh assign their own arrays; k_cc_gather
+    // adds k_cc_work's slots to the cores after the join
     hipStream_t qt = q, ql = q, qc = q;
     if (fs) {
         (void)hipEventRecord(fs->fork, q);
@@ -1049,10 +1069,15 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
         M(K_CCWORK, 1);
     }
     M(K_MECH, 0); k_mech<<<nblk(s.n, 64), 64, 0, q>>>(s, g, f, E(off_mech), terms); M(K_MECH, 1);
-    if (s.n_lnk) { M(K_LINKER, 0); k_linker<<<s.n_lnk, kRecvThreads, 0, q>>>(s, g, st, f, E(off_lnk), terms); M(K_LINKER, 1); }
+    if (fs) {
+        (void)hipEventRecord(fs->mech, q);
+        (void)hipStreamWaitEvent(ql, fs->mech, 0);
+    }
+    if (s.n_lnk) { M(K_LINKER, 0); k_linker<<<s.n_lnk, kRecvThreads, 0, ql>>>(s, g, st, f, E(off_lnk), terms); M(K_LINKER, 1); }
     M(K_CORE, 0);
-    k_core<<<dim3(s.n_c, w.core_split), kCoreThreads, 0, q>>>(s, g, st, f, w.core_part, w.core_cnt, E(off_core),
-                                                            terms);
+    const int *cs = w.core_split;
+    k_core<<<dim3(s.n_c, cs[0] + cs[1] + cs[2]), kCoreThreads, 0, q>>>(s, g, st, f, w.core_part, w.core_cnt,
+                                                                      E(off_core), terms, cs[0], cs[1]);
     M(K_CORE, 1);
     if (fs) {
         for (int i = 0; i < 3; i++) {

@@ -12,10 +12,11 @@ struct ForceScratch {
     double *cc_slots;         // n_c * n_c * nchunk * 10
     int cc_nchunk;
     int cc_grid;
-    // k_core runs core_split workgroups per core; each leaves 8 partial sums
-    // in core_part and the last to finish (core_cnt) adds them up in order
-    int core_split;
-    double *core_part;        // n_c * core_split * 8
+    // k_core runs core_split[p] workgroups per core on its phase p; each
+    // leaves 8 partial sums in core_part and the last to finish (core_cnt)
+    // adds them up in order
+    int core_split[3];
+    double *core_part;        // n_c * (sum of core_split) * 8
     int *core_cnt;            // n_c, zero between passes
 };
 
@@ -28,11 +29,12 @@ enum KernelId { K_SITES, K_MECH, K_LINKER, K_TAIL, K_LH, K_CORE, K_CCLIST, K_CCW
                 K_AVG, K_COMMIT, K_EREDUCE, K_CCGATHER, K_COUNT };
 typedef void (*MarkFn)(void *, int, int);
 // Extra streams of a force pass: after the charge sites, the tail, the
-// linker-histone and the core-core branches run beside mechanics + linker
-// on the pass's stream, and the core kernel waits for all of them.
+// linker-histone (then, once mechanics is done, linker) and the core-core
+// branches run beside mechanics + core on the pass's stream, and the
+// core-core gather waits for all of them.
 struct ForceStreams {
     hipStream_t b[3];
-    hipEvent_t fork, join[3];
+    hipEvent_t fork, mech, join[3];
 };
 void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, const ForceScratch &w,
                    double *Epart, int terms, hipStream_t q, MarkFn mark = nullptr, void *arg = nullptr,
