@@ -110,25 +110,26 @@ __device__ __forceinline__ double rsqrt_f64(double f) {
 // trailing zero bits and the low remainder, the published fdlibm constants,
 // so that n ln2_hi is exact), e^r by its Taylor series to r^12 (truncation
 // below 2e-16 relative) in Horner form, scaled by 2^n.
-__device__ __forceinline__ double exp_neg(double x) {
+// Taylor coefficients 1/k! of e^r for k = 12 .. 2.
+struct ExpCoef {
+    double c[11];
+};
+constexpr ExpCoef kExpCoef = {{1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0, 1.0 / 362880.0,
+                               1.0 / 40320.0, 1.0 / 5040.0, 1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0,
+                               0.5}};
+template <typename C>
+__device__ __forceinline__ double exp_neg_c(double x, const C &k) {
     const double n = rint(x * 1.4426950408889634);
     double r = __builtin_fma(-n, 6.93147180369123816490e-01, x);
     r = __builtin_fma(-n, 1.90821492927058770002e-10, r);
-    double p = 1.0 / 479001600.0;
-    p = __builtin_fma(p, r, 1.0 / 39916800.0);
-    p = __builtin_fma(p, r, 1.0 / 3628800.0);
-    p = __builtin_fma(p, r, 1.0 / 362880.0);
-    p = __builtin_fma(p, r, 1.0 / 40320.0);
-    p = __builtin_fma(p, r, 1.0 / 5040.0);
-    p = __builtin_fma(p, r, 1.0 / 720.0);
-    p = __builtin_fma(p, r, 1.0 / 120.0);
-    p = __builtin_fma(p, r, 1.0 / 24.0);
-    p = __builtin_fma(p, r, 1.0 / 6.0);
-    p = __builtin_fma(p, r, 0.5);
+    double p = k.c[0];
+#pragma unroll
+    for (int i = 1; i < 11; i++) p = __builtin_fma(p, r, k.c[i]);
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
     return __builtin_ldexp(p, (int)n);
 }
+__device__ __forceinline__ double exp_neg(double x) { return exp_neg_c(x, kExpCoef); }
 __device__ __forceinline__ double3 zero3() { return make_double3(0.0, 0.0, 0.0); }
 
 // r + a*p.x + b*p.y + c*p.z (body point p of a frame at r)

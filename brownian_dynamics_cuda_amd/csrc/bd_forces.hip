@@ -935,9 +935,13 @@ constexpr int kCCChunkMax = 160;
 // (3), E}.  The torque on c2 is summed in world axes about its centre and
 // turned into body components by k_core (equal to the reference's
 // body-frame sum for an orthonormal frame).
+// The Taylor coefficients of e^r come in as a kernel argument, so they sit
+// in scalar registers and each Horner step is one FMA with a scalar addend
+// (compile-time constants end up in vector registers, where every step
+// first copies its addend).
 template <bool WITH_E>
 __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, Sites st, const int *list,
-                                                       const int *count, int nchunk, double *slots) {
+                                                       const int *count, int nchunk, double *slots, ExpCoef ec) {
     __shared__ double sq[kCCChunkMax * 7];
     __shared__ double red[10 * (kCCThreads / 64)];
     const int nunits = (*count) * nchunk;
@@ -974,7 +978,7 @@ __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, Sites st,
                 const double r2s = dot(z, z);
                 const double inv = rsqrt_f64(r2s);
                 const double dist = r2s * inv;
-                const double t = keq * d[3] * exp_neg(-kappa * dist);
+                const double t = keq * d[3] * exp_neg_c(-kappa * dist, ec);
                 if (WITH_E) E += t * inv;
                 double g1d = t * (kappa * dist + 1.0) * (inv * inv * inv);
                 if (dist <= 8.0) {
@@ -1063,9 +1067,11 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
         M(K_CCLIST, 1);
         M(K_CCWORK, 0);
         if (Epart)
-            k_cc_work<true><<<w.cc_grid, kCCThreads, 0, qc>>>(s, g, st, w.cc_list, w.cc_count, w.cc_nchunk, w.cc_slots);
+            k_cc_work<true><<<w.cc_grid, kCCThreads, 0, qc>>>(s, g, st, w.cc_list, w.cc_count, w.cc_nchunk, w.cc_slots,
+                                                                   kExpCoef);
         else
-            k_cc_work<false><<<w.cc_grid, kCCThreads, 0, qc>>>(s, g, st, w.cc_list, w.cc_count, w.cc_nchunk, w.cc_slots);
+            k_cc_work<false><<<w.cc_grid, kCCThreads, 0, qc>>>(s, g, st, w.cc_list, w.cc_count, w.cc_nchunk, w.cc_slots,
+                                                                   kExpCoef);
         M(K_CCWORK, 1);
     }
     M(K_MECH, 0); k_mech<<<nblk(s.n, 64), 64, 0, q>>>(s, g, f, E(off_mech), terms); M(K_MECH, 1);
