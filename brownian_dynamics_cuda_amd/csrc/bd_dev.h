@@ -73,7 +73,7 @@ struct Stat {
 };
 
 // partners per bounding sphere of the culled partner loops
-constexpr int kChunk = 32;
+constexpr int kChunk = 16;
 __host__ __device__ inline int nchunks(int n) { return (n + kChunk - 1) / kChunk; }
 
 struct Sites {
@@ -130,6 +130,45 @@ __device__ __forceinline__ double exp_neg_c(double x, const C &k) {
     return __builtin_ldexp(p, (int)n);
 }
 __device__ __forceinline__ double exp_neg(double x) { return exp_neg_c(x, kExpCoef); }
+// sin and cos of x for |x| < 2^19 (the frame angles here lie within
+// [-2 pi, 2 pi]), branch-free: x = n pi/2 + r with n = rint(2x/pi) and pi/2
+// in three parts (the published constants of fdlibm's e_rem_pio2.c: two
+// 33-bit parts, so that n pio2_1 and n pio2_2 are exact, and a 53-bit
+// tail), |r| <= pi/4; Taylor series of sin r to r^15 and cos r to r^16
+// (truncation below 7e-17) in Horner form in r^2; then the quadrant n mod 4.
+__host__ __device__ __forceinline__ void sincos_small(double x, double &sn, double &cs) {
+    const double n = rint(x * 6.36619772367581382433e-01);
+    double r = __builtin_fma(-n, 1.57079632673412561417e+00, x);
+    r = __builtin_fma(-n, 6.07710050630396597660e-11, r);
+    r = __builtin_fma(-n, 2.02226624879595063154e-21, r);
+    const double z = r * r;
+    double ps = -1.0 / 1307674368000.0;
+    ps = __builtin_fma(ps, z, 1.0 / 6227020800.0);
+    ps = __builtin_fma(ps, z, -1.0 / 39916800.0);
+    ps = __builtin_fma(ps, z, 1.0 / 362880.0);
+    ps = __builtin_fma(ps, z, -1.0 / 5040.0);
+    ps = __builtin_fma(ps, z, 1.0 / 120.0);
+    ps = __builtin_fma(ps, z, -1.0 / 6.0);
+    const double sr = __builtin_fma(r * z, ps, r);
+    double pc = 1.0 / 20922789888000.0;
+    pc = __builtin_fma(pc, z, -1.0 / 87178291200.0);
+    pc = __builtin_fma(pc, z, 1.0 / 479001600.0);
+    pc = __builtin_fma(pc, z, -1.0 / 3628800.0);
+    pc = __builtin_fma(pc, z, 1.0 / 40320.0);
+    pc = __builtin_fma(pc, z, -1.0 / 720.0);
+    pc = __builtin_fma(pc, z, 1.0 / 24.0);
+    pc = __builtin_fma(pc, z, -0.5);
+    const double cr = __builtin_fma(z, pc, 1.0);
+    const int q = (int)n & 3;
+    const double s0 = (q & 1) ? cr : sr, c0 = (q & 1) ? sr : cr;
+    sn = (q & 2) ? -s0 : s0;
+    cs = ((q + 1) & 2) ? -c0 : c0;
+}
+__device__ __forceinline__ double tan_small(double x) {
+    double sn, cs;
+    sincos_small(x, sn, cs);
+    return sn / cs;
+}
 __device__ __forceinline__ double3 zero3() { return make_double3(0.0, 0.0, 0.0); }
 
 // r + a*p.x + b*p.y + c*p.z (body point p of a frame at r)

@@ -89,10 +89,11 @@ __device__ void segment(const Stat &s, const Geo &g, int i, SegOut &o) {
     {
         double bb = g.beta[i], len = g.length[i];
         o.E += s.g * (bb - beta_b) * (bb - beta_b) / 2;
-        double g1 = (bb >= 1e-10) ? bb / (sin(bb) * len) : 1.0 / len;
-        double c1 = cos(bb);
-        double g2 = (beta_b >= 1e-10) ? beta_b / (sin(beta_b) * len) : 1.0 / len;
-        double c2 = cos(beta_b);
+        double s1, c1, s2, c2;
+        sincos_small(bb, s1, c1);
+        sincos_small(beta_b, s2, c2);
+        double g1 = (bb >= 1e-10) ? bb / (s1 * len) : 1.0 / len;
+        double g2 = (beta_b >= 1e-10) ? beta_b / (s2 * len) : 1.0 / len;
         double3 A = make_double3(g1 * (af.x - c1 * ao.x), g1 * (af.y - c1 * ao.y), g1 * (af.z - c1 * ao.z));
         double3 B = make_double3(g2 * (ab.x - c2 * ao.x), g2 * (ab.y - c2 * ao.y), g2 * (ab.z - c2 * ao.z));
         o.b1 = make_double3(-s.g * (A.x + B.x), -s.g * (A.y + B.y), -s.g * (A.z + B.z));
@@ -113,10 +114,11 @@ __device__ void segment(const Stat &s, const Geo &g, int i, SegOut &o) {
         }
         double tw = g.alpha[i] + g.gamma[i] - s.phi_o[i];
         o.E += s.s * (tw * tw) / 2;
-        double g1 = tw * tan(0.5 * g.beta[i]) / g.length[i];
-        double ca = cos(g.alpha[i]), sa = sin(g.alpha[i]);
-        double g2 = (al_b + ga_b - ph_b) * tan(0.5 * be_b) / g.length[i];
-        double cg = cos(ga_n), sg = sin(ga_n);
+        double g1 = tw * tan_small(0.5 * g.beta[i]) / g.length[i];
+        double ca, sa, cg, sg;
+        sincos_small(g.alpha[i], sa, ca);
+        double g2 = (al_b + ga_b - ph_b) * tan_small(0.5 * be_b) / g.length[i];
+        sincos_small(ga_n, sg, cg);
         double3 ch = make_double3(g1 * (ca * cc.x - sa * bo.x), g1 * (ca * cc.y - sa * bo.y), g1 * (ca * cc.z - sa * bo.z));
         double3 zh = make_double3(g2 * (cg * cc.x + sg * bo.x), g2 * (cg * cc.y + sg * bo.y), g2 * (cg * cc.z + sg * bo.z));
         o.t1 = make_double3(s.s * (ch.x + zh.x), s.s * (ch.y + zh.y), s.s * (ch.z + zh.z));
@@ -277,26 +279,28 @@ __global__ void k_mech(Stat s, Geo g, Frc f, double *Epart, int terms) {
                 mag = -s.s * (al[j - 1] + ga[j - 1] - ph[j - 1]);
                 T.x += mag * co; T.y += mag * si;
             }
-            double bp = g.beta_p[ic];
-            T.y = T.y - s.g * bp * adc / sin(bp);
-            T.z = T.z + s.g * bp * adb / sin(bp);
+            double bp = g.beta_p[ic], sbp, cbp;
+            sincos_small(bp, sbp, cbp);
+            T.y = T.y - s.g * bp * adc / sbp;
+            T.z = T.z + s.g * bp * adb / sbp;
             if (j > 0) {
                 double3 ab = ld3(g.a + 3 * j - 3);
                 double xa = dot(ab, aj), xb = dot(ab, bj), xc = dot(ab, cj);
-                double bm = g.beta[j - 1];
-                T.x = T.x + s.g * bm * (si * xc) / sin(bm);
-                T.y = T.y - s.g * bm * (co * xc) / sin(bm);
-                T.z = T.z + s.g * bm * (co * xb - si * xa) / sin(bm);
+                double bm = g.beta[j - 1], sbm, cbm;
+                sincos_small(bm, sbm, cbm);
+                T.x = T.x + s.g * bm * (si * xc) / sbm;
+                T.y = T.y - s.g * bm * (co * xc) / sbm;
+                T.z = T.z + s.g * bm * (co * xb - si * xa) / sbm;
             }
-            double s1 = sin(g.alpha_p[ic]), c1 = cos(g.alpha_p[ic]);
-            mag = s.s * (al[j] + ga[j] - ph[j]) * tan(0.5 * bp);
+            double s1, c1;
+            sincos_small(g.alpha_p[ic], s1, c1);
+            mag = s.s * (al[j] + ga[j] - ph[j]) * tan_small(0.5 * bp);
             T.x = T.x - mag * (s1 * dot(cd, aj) + c1 * dot(bd, aj));
             T.y = T.y - mag * (s1 * dot(cd, bj) + c1 * dot(bd, bj));
             T.z = T.z - mag * (s1 * dot(cd, cj) + c1 * dot(bd, cj));
             if (j > 0) {
-                s1 = sin(ga[j - 1]);
-                c1 = cos(ga[j - 1]);
-                mag = s.s * (al[j - 1] + ga[j - 1] - ph[j - 1]) * tan(0.5 * g.beta[j - 1]);
+                sincos_small(ga[j - 1], s1, c1);
+                mag = s.s * (al[j - 1] + ga[j - 1] - ph[j - 1]) * tan_small(0.5 * g.beta[j - 1]);
                 T.x = T.x - mag * (s1 * 0.0 - c1 * (-si));
                 T.y = T.y - mag * (s1 * 0.0 - c1 * co);
                 T.z = T.z - mag * (s1 * 1.0 - c1 * 0.0);
