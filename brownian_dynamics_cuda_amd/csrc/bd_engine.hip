@@ -467,22 +467,25 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     if (alloc_geo(c, c->cur) || alloc_geo(c, c->pred)) return bail(BD_E_HIP);
     if (alloc_frc(c, c->f0) || alloc_frc(c, c->f1) || alloc_frc(c, c->fa)) return bail(BD_E_HIP);
     r |= c->alloc(&c->sites.Q, 3 * (size_t)n_c * Nq);
+    r |= c->alloc(&c->sites.Qp, 8 * (size_t)n_c * Nq);
     r |= c->alloc(&c->sites.G, 3 * (size_t)n_c * std::max(sys->n_lh_g, 1));
     r |= c->alloc(&c->sites.ch_t, nchunks(nt));
     r |= c->alloc(&c->sites.ch_lh, nchunks(c->lh_cap));
     r |= c->alloc(&c->sites.ch_g, nchunks(n_c * std::max(sys->n_lh_g, 1)));
     r |= c->alloc(&c->sites.ch_l, nchunks(n));
     // core-core scratch
-    c->fw.cc_nchunk = 2;
-    while ((Nq + c->fw.cc_nchunk - 1) / c->fw.cc_nchunk > 160) c->fw.cc_nchunk++;
+    // one wavefront per (pair, 64 charges of the first core, about 80
+    // charges of the second)
+    c->fw.cc_nkb = (Nq + 63) / 64;
+    c->fw.cc_nlc = std::max(1, (Nq + 79) / 80);
     {
         long long npairs = (long long)n_c * (n_c - 1) / 2;
-        long long units = npairs * c->fw.cc_nchunk;
-        c->fw.cc_grid = (int)std::max(1LL, std::min(units, 4096LL));
+        long long units = npairs * c->fw.cc_nkb * c->fw.cc_nlc;
+        c->fw.cc_grid = (int)std::max(1LL, std::min((units + 3) / 4, 2048LL));
         r |= c->alloc(&c->fw.cc_valid, (size_t)n_c * n_c);
         r |= c->alloc(&c->fw.cc_list, 2 * (size_t)std::max(npairs, 1LL));
         r |= c->alloc(&c->fw.cc_count, 1);
-        r |= c->alloc(&c->fw.cc_slots, (size_t)n_c * n_c * c->fw.cc_nchunk * 10);
+        r |= c->alloc(&c->fw.cc_slots, (size_t)n_c * n_c * c->fw.cc_nkb * c->fw.cc_nlc * 10);
         c->fw.core_split[0] = 1;
         c->fw.core_split[1] = 2;
         c->fw.core_split[2] = 1;

@@ -26,6 +26,10 @@ __global__ void k_sites(Stat s, Geo g, Sites st) {
         double3 w = body_to_world(ld3(g.r + 3 * cb), ld3(g.a + 3 * cb), ld3(g.b + 3 * cb),
                                   ld3(g.c + 3 * cb), ld3(s.cpos + 3 * q));
         st3(st.Q + 3 * t, w);
+        const double3 rc = ld3(g.r + 3 * cb);
+        double *p = st.Qp + 8 * (size_t)t;
+        p[0] = w.x; p[1] = w.y; p[2] = w.z; p[3] = s.cq[q];
+        p[4] = w.x - rc.x; p[5] = w.y - rc.y; p[6] = w.z - rc.z; p[7] = 0.0;
         return;
     }
     t -= nq;
@@ -931,84 +935,83 @@ __global__ void k_cc_list(Stat s, Geo g, unsigned char *valid, int *list, int *c
     }
 }
 
-constexpr int kCCThreads = 320;
-constexpr int kCCChunkMax = 160;
+constexpr int kCCThreads = 256;   // four wavefronts, a work unit each
 
-// Work unit = (pair, chunk of the second core's charges).  Thread = charge
-// of the first core.  Slot = {F on c1 (3), T on c1 body (3), T on c2 world
-// (3), E}.  The torque on c2 is summed in world axes about its centre and
-// turned into body components by k_core (equal to the reference's
-// body-frame sum for an orthonormal frame).
+// Work unit = (core pair, block of 64 charges of the first core, chunk of
+// the second core's charges), one wavefront: lane = charge k of the first
+// core; the second core's charge records are the same for every lane and
+// come through scalar loads (no LDS, no barriers).  Slot per unit = {F on
+// c1 (3), T on c1 body (3), T on c2 world (3), E}.  The torque on c2 is
+// summed in world axes about its centre and turned into body components by
+// k_cc_gather (equal to the reference's body-frame sum for an orthonormal
+// frame).  Lanes past the last charge repeat it and are weighted by 0.
 // The Taylor coefficients of e^r come in as a kernel argument, so they sit
 // in scalar registers and each Horner step is one FMA with a scalar addend
 // (compile-time constants end up in vector registers, where every step
 // first copies its addend).
 template <bool WITH_E>
-__global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, Sites st, const int *list,
-                                                       const int *count, int nchunk, double *slots, ExpCoef ec) {
-    __shared__ double sq[kCCChunkMax * 7];
-    __shared__ double red[10 * (kCCThreads / 64)];
-    const int nunits = (*count) * nchunk;
-    const int chunk = (s.Nq + nchunk - 1) / nchunk;
+__global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, const double *__restrict__ Qp,
+                                                       const int *__restrict__ list, const int *__restrict__ count,
+                                                       int nkb, int nlc, double *__restrict__ slots, ExpCoef ec) {
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nsub = nkb * nlc, nunits = (*count) * nsub, Nq = s.Nq;
+    const int chunk = (Nq + nlc - 1) / nlc;
     const double kappa = s.debye, kex = s.k_ex, sg = s.sig_cc, c12 = 12.0 / sg, c6 = 6.0 / sg;
-    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
-        const int pr = u / nchunk, ch = u % nchunk;
+    for (int u = blockIdx.x * (kCCThreads / 64) + wv; u < nunits; u += gridDim.x * (kCCThreads / 64)) {
+        const int pr = u / nsub, sub = u % nsub, kb = sub / nlc, lc = sub % nlc;
         const int k1 = list[2 * pr], k2 = list[2 * pr + 1];
-        const int b1 = s.core_bead[k1], b2 = s.core_bead[k2];
-        const int l0 = ch * chunk, l1 = min(s.Nq, l0 + chunk);
-        const double3 r2 = ld3(g.r + 3 * b2);
-        __syncthreads();
-        for (int l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
-            double3 Q = ld3(st.Q + 3 * ((size_t)k2 * s.Nq + l));
-            double *d = sq + 7 * (l - l0);
-            d[0] = Q.x; d[1] = Q.y; d[2] = Q.z; d[3] = s.cq[l];
-            d[4] = Q.x - r2.x; d[5] = Q.y - r2.y; d[6] = Q.z - r2.z;
-        }
-        __syncthreads();
-        double3 F = zero3(), T1 = zero3(), T2 = zero3();
+        const int b1 = s.core_bead[k1];
+        const int k = min(kb * 64 + lane, Nq - 1);
+        const double act = (kb * 64 + lane < Nq) ? 1.0 : 0.0;
+        const int l0 = lc * chunk, l1 = min(Nq, l0 + chunk);
+        const double *pk = Qp + 8 * ((size_t)k1 * Nq + k);
+        const double3 Qk = make_double3(pk[0], pk[1], pk[2]);
+        const double keq = s.k_e * pk[3];
+        // Fk = sum_l f_l and Gk = sum_l g_l (Q_l - r2), f_l = g_l (Qk - Q_l)
+        double3 Fk = zero3(), Gk = zero3();
         double E = 0.0;
-        const double3 a1 = ld3(g.a + 3 * b1), bb1 = ld3(g.b + 3 * b1), c1 = ld3(g.c + 3 * b1);
-        for (int k = threadIdx.x; k < s.Nq; k += blockDim.x) {
-            const double3 Qk = ld3(st.Q + 3 * ((size_t)k1 * s.Nq + k));
-            const double keq = s.k_e * s.cq[k];
-            // Fk = sum_l f_l and Gk = sum_l g_l (Q_l - r2), f_l = g_l (Qk - Q_l)
-            double3 Fk = zero3(), Gk = zero3();
-            const int nl = l1 - l0;
-            for (int l = 0; l < nl; l++) {
-                const double *d = sq + 7 * l;
-                const double3 z = make_double3(Qk.x - d[0], Qk.y - d[1], Qk.z - d[2]);
-                // Debye-Hueckel + excluded volume of the reference (dh_core, ev_core)
-                // with 1/d from a refined reciprocal square root
-                const double r2s = dot(z, z);
-                const double inv = rsqrt_f64(r2s);
-                const double dist = r2s * inv;
-                const double t = keq * d[3] * exp_neg_c(-kappa * dist, ec);
-                if (WITH_E) E += t * inv;
-                double g1d = t * (kappa * dist + 1.0) * (inv * inv * inv);
-                if (dist <= 8.0) {
-                    const double x = sg * inv, x2 = x * x, x6 = x2 * x2 * x2, x7 = x6 * x, x13 = x6 * x7;
-                    if (WITH_E) E += kex * (x6 * x6 - x6);
-                    g1d += kex * (c12 * x13 - c6 * x7) * inv;
-                }
-                Fk.x = __builtin_fma(g1d, z.x, Fk.x);
-                Fk.y = __builtin_fma(g1d, z.y, Fk.y);
-                Fk.z = __builtin_fma(g1d, z.z, Fk.z);
-                Gk.x = __builtin_fma(g1d, d[4], Gk.x);
-                Gk.y = __builtin_fma(g1d, d[5], Gk.y);
-                Gk.z = __builtin_fma(g1d, d[6], Gk.z);
+        const double *pl = Qp + 8 * (size_t)k2 * Nq;
+        for (int l = l0; l < l1; l++) {
+            const double *d = pl + 8 * l;
+            const double3 z = make_double3(Qk.x - d[0], Qk.y - d[1], Qk.z - d[2]);
+            // Debye-Hueckel + excluded volume of the reference (dh_core, ev_core)
+            // with 1/d from a refined reciprocal square root
+            const double r2s = dot(z, z);
+            const double inv = rsqrt_f64(r2s);
+            const double dist = r2s * inv;
+            const double t = keq * d[3] * exp_neg_c(-kappa * dist, ec);
+            if (WITH_E) E += t * inv;
+            double g1d = t * (kappa * dist + 1.0) * (inv * inv * inv);
+            if (dist <= 8.0) {
+                const double x = sg * inv, x2 = x * x, x6 = x2 * x2 * x2, x7 = x6 * x, x13 = x6 * x7;
+                if (WITH_E) E += kex * (x6 * x6 - x6);
+                g1d += kex * (c12 * x13 - c6 * x7) * inv;
             }
-            F += Fk;
-            T1 += torque_rel(Fk, ld3(s.cpos + 3 * k), a1, bb1, c1);
-            // torque on c2 about its centre from -f_l at lever (Q_l - r2):
-            // sum_l (Q_l - r2) x (-g_l (Qk - Q_l)) = -Gk x (Qk - r2), since
-            // (Q_l - r2) x (Q_l - r2) = 0
-            T2 -= cross(Gk, Qk - r2);
+            Fk.x = __builtin_fma(g1d, z.x, Fk.x);
+            Fk.y = __builtin_fma(g1d, z.y, Fk.y);
+            Fk.z = __builtin_fma(g1d, z.z, Fk.z);
+            Gk.x = __builtin_fma(g1d, d[4], Gk.x);
+            Gk.y = __builtin_fma(g1d, d[5], Gk.y);
+            Gk.z = __builtin_fma(g1d, d[6], Gk.z);
         }
-        double v[10] = {F.x, F.y, F.z, T1.x, T1.y, T1.z, T2.x, T2.y, T2.z, E};
-        block_sum_n<kCCThreads, 10>(v, red);
-        double *slot = slots + 10 * ((size_t)(k1 * s.n_c + k2) * nchunk + ch);
-        if (threadIdx.x == 0)
+        Fk = act * Fk;
+        Gk = act * Gk;
+        const double3 a1 = ld3(g.a + 3 * b1), bb1 = ld3(g.b + 3 * b1), c1 = ld3(g.c + 3 * b1);
+        const double3 T1 = torque_rel(Fk, ld3(s.cpos + 3 * k), a1, bb1, c1);
+        // torque on c2 about its centre from -f_l at lever (Q_l - r2):
+        // sum_l (Q_l - r2) x (-g_l (Qk - Q_l)) = -Gk x (Qk - r2), since
+        // (Q_l - r2) x (Q_l - r2) = 0
+        const double3 T2 = cross(Gk, Qk - ld3(g.r + 3 * s.core_bead[k2]));
+        double v[10] = {Fk.x, Fk.y, Fk.z, T1.x, T1.y, T1.z, -T2.x, -T2.y, -T2.z, act * E};
+#pragma unroll
+        for (int q = 0; q < 10; q++)
+            for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
+        if (lane == 0) {
+            double *slot = slots + 10 * ((size_t)(k1 * s.n_c + k2) * nsub + sub);
+#pragma unroll
             for (int q = 0; q < 10; q++) slot[q] = v[q];
+        }
     }
 }
 
@@ -1071,11 +1074,11 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
         M(K_CCLIST, 1);
         M(K_CCWORK, 0);
         if (Epart)
-            k_cc_work<true><<<w.cc_grid, kCCThreads, 0, qc>>>(s, g, st, w.cc_list, w.cc_count, w.cc_nchunk, w.cc_slots,
-                                                                   kExpCoef);
+            k_cc_work<true><<<w.cc_grid, kCCThreads, 0, qc>>>(s, g, st.Qp, w.cc_list, w.cc_count, w.cc_nkb,
+                                                              w.cc_nlc, w.cc_slots, kExpCoef);
         else
-            k_cc_work<false><<<w.cc_grid, kCCThreads, 0, qc>>>(s, g, st, w.cc_list, w.cc_count, w.cc_nchunk, w.cc_slots,
-                                                                   kExpCoef);
+            k_cc_work<false><<<w.cc_grid, kCCThreads, 0, qc>>>(s, g, st.Qp, w.cc_list, w.cc_count, w.cc_nkb,
+                                                               w.cc_nlc, w.cc_slots, kExpCoef);
         M(K_CCWORK, 1);
     }
     M(K_MECH, 0); k_mech<<<nblk(s.n, 64), 64, 0, q>>>(s, g, f, E(off_mech), terms); M(K_MECH, 1);
@@ -1096,7 +1099,7 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
         }
     }
     M(K_CCGATHER, 0);
-    k_cc_gather<<<s.n_c, 64, 0, q>>>(s, g, f, w.cc_valid, w.cc_nchunk, w.cc_slots, E(off_cc), terms);
+    k_cc_gather<<<s.n_c, 64, 0, q>>>(s, g, f, w.cc_valid, w.cc_nkb * w.cc_nlc, w.cc_slots, E(off_cc), terms);
     M(K_CCGATHER, 1);
 }
 

@@ -9,8 +9,8 @@ struct ForceScratch {
     unsigned char *cc_valid;  // n_c * n_c
     int *cc_list;             // 2 * n_c * (n_c - 1) / 2
     int *cc_count;            // 1
-    double *cc_slots;         // n_c * n_c * nchunk * 10
-    int cc_nchunk;
+    double *cc_slots;         // n_c * n_c * nkb * nlc * 10
+    int cc_nkb, cc_nlc;       // work units per pair: 64-charge blocks x chunks
     int cc_grid;
     // k_core runs core_split[p] workgroups per core on its phase p; each
     // leaves 8 partial sums in core_part and the last to finish (core_cnt)
