@@ -27,8 +27,11 @@ typedef float flt4 __attribute__((ext_vector_type(4)));
 static inline int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
 
 // ---------------------------------------------------------------------------
-// RPY tensor assembly (func.cpp:737-804): one element per thread, lower
-// triangle only, identity padding.
+// RPY tensor assembly (func.cpp:737-804): one workgroup per lower 64x64
+// tile, a thread per bead pair whose 3x3 block meets the tile (the pair's
+// distance terms evaluated once, each entry by the reference's expression),
+// staged in LDS and stored row by row; zeros above the diagonal and the
+// identity padding.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double3 hi_pos(const Stat &s, const Geo &g, int b) {
     if (b < s.n) return ld3(g.r + 3 * b);
@@ -37,20 +40,30 @@ __device__ __forceinline__ double3 hi_pos(const Stat &s, const Geo &g, int b) {
     return ld3(g.r_lh + 3 * (b - s.n_tail));
 }
 
-__device__ __forceinline__ double rpy_entry(double3 ri, double3 rj, double radi, double radj, int k, int l,
-                                            double a1, double a2) {
-    double sij[3] = {ri.x - rj.x, ri.y - rj.y, ri.z - rj.z};
-    double ssq = sij[0] * sij[0] + sij[1] * sij[1] + sij[2] * sij[2];
-    double s = sqrt(ssq);
+// M[k][l] for beads i != j at ri, rj (func.cpp:754-796)
+__device__ __forceinline__ void rpy_block(double3 ri, double3 rj, double radi, double radj, double a1, double a2,
+                                          double (&M)[3][3]) {
+    const double sij[3] = {ri.x - rj.x, ri.y - rj.y, ri.z - rj.z};
+    const double ssq = sij[0] * sij[0] + sij[1] * sij[1] + sij[2] * sij[2];
+    const double s = sqrt(ssq);
     if (s >= radi + radj) {
-        double f = (radi * radi + radj * radj) / ssq;
-        double f1 = 1.0 + f / 3, f2 = 1.0 - f;
-        return (k == l) ? (a2 / s) * (f1 + f2 * sij[k] * sij[k] / ssq) : (a2 / s) * (f2 * sij[k] * sij[l] / ssq);
+        const double f = (radi * radi + radj * radj) / ssq;
+        const double f1 = 1.0 + f / 3, f2 = 1.0 - f, c = a2 / s;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int l = 0; l < 3; l++)
+                M[k][l] = (k == l) ? c * (f1 + f2 * sij[k] * sij[k] / ssq) : c * (f2 * sij[k] * sij[l] / ssq);
+    } else {
+        const double se = pow((radi * radi * radi + radj * radj * radj) / 2.0, 1.0 / 3);
+        const double sq = sqrt(ssq), c = a1 / se;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int l = 0; l < 3; l++)
+                M[k][l] = (k == l) ? c * (1.0 - 9.0 * sq / (32 * se) + 3.0 * sij[k] * sij[k] / (32.0 * se * sq))
+                                   : c * (3.0 * sij[k] * sij[l] / (32.0 * se * sq));
     }
-    double se = pow((radi * radi * radi + radj * radj * radj) / 2.0, 1.0 / 3);
-    double sq = sqrt(ssq);
-    return (k == l) ? (a1 / se) * (1.0 - 9.0 * sq / (32 * se) + 3.0 * sij[k] * sij[k] / (32.0 * se * sq))
-                    : (a1 / se) * (3.0 * sij[k] * sij[l] / (32.0 * se * sq));
 }
 
 __device__ __forceinline__ void tri_index(int idx, int &i, int &j) {
@@ -64,18 +77,44 @@ __device__ __forceinline__ void tri_index(int idx, int &i, int &j) {
 
 __global__ void __launch_bounds__(256) k_rpy(Stat s, Geo g, const double *rad_all, double *A, int m, int ld,
                                              double a1, double a2) {
+    __shared__ double T[TB * LDP];
     int ti, tj;
     tri_index(blockIdx.x, ti, tj);
+    const int r0 = ti * TB, c0 = tj * TB, nhi = m / 3;
+    // beads whose rows / columns meet the tile
+    const int bi0 = r0 / 3, bj0 = c0 / 3;
+    const int nbi = max(0, min((r0 + TB - 1) / 3, nhi - 1) - bi0 + 1);
+    const int nbj = max(0, min((c0 + TB - 1) / 3, nhi - 1) - bj0 + 1);
+    for (int p = threadIdx.x; p < nbi * nbj; p += blockDim.x) {
+        const int i = bi0 + p / nbj, j = bj0 + p % nbj;
+        if (j > i) continue;
+        double M[3][3];
+        if (i == j) {
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int l = 0; l < 3; l++) M[k][l] = (k == l) ? a1 / rad_all[i] : 0.0;
+        } else {
+            rpy_block(hi_pos(s, g, i), hi_pos(s, g, j), rad_all[i], rad_all[j], a1, a2, M);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int row = 3 * i + k - r0;
+            if (row < 0 || row >= TB) continue;
+#pragma unroll
+            for (int l = 0; l < 3; l++) {
+                const int col = 3 * j + l - c0;
+                if (col >= 0 && col < TB) T[row * LDP + col] = M[k][l];
+            }
+        }
+    }
+    __syncthreads();
     for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
-        const int row = ti * TB + e / TB, col = tj * TB + e % TB;
+        const int row = r0 + e / TB, col = c0 + e % TB;
         double v;
         if (col > row) v = 0.0;
         else if (row >= m || col >= m) v = (row == col) ? 1.0 : 0.0;
-        else {
-            const int i = row / 3, j = col / 3, k = row % 3, l = col % 3;
-            if (i == j) v = (k == l) ? a1 / rad_all[i] : 0.0;
-            else v = rpy_entry(hi_pos(s, g, i), hi_pos(s, g, j), rad_all[i], rad_all[j], k, l, a1, a2);
-        }
+        else v = T[(e / TB) * LDP + e % TB];
         A[(size_t)row * ld + col] = v;
     }
 }

@@ -336,14 +336,17 @@ __device__ __forceinline__ double ev_core(double kex, double sg, double d, doubl
 // receiving bead, its threads split the partner lists, and a fixed-order
 // block reduction forms the bead's force (deterministic, no atomics).
 // ---------------------------------------------------------------------------
-constexpr int kRecvThreads = 256;
+// Workgroup sizes: a tail bead has few partners within reach and many
+// receivers run side by side (one wavefront each); linker beads carry the
+// most partners (DNA and core charges).
+constexpr int kTailThreads = 64, kLHThreads = 128, kLinkerThreads = 256;
 
 // Linker bead (func.cpp:1304-1487 linker side, 1898-1917, 2249-2279,
 // 2481-2522): DNA-DNA, DNA-core charges, tail-linker, LH-DNA, LHG-DNA.
 // F[j] +=.  Energy owner: DNA-DNA pairs with partner index > j.
-__global__ void __launch_bounds__(kRecvThreads) k_linker(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
-    __shared__ double red[4 * (kRecvThreads / 64)];
-    __shared__ int s_list[kRecvThreads], s_wsum[kRecvThreads / 64];
+__global__ void __launch_bounds__(kLinkerThreads) k_linker(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
+    __shared__ double red[4 * (kLinkerThreads / 64)];
+    __shared__ int s_list[kLinkerThreads], s_wsum[kLinkerThreads / 64];
     const int li = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
     const int j = s.lnk[li];
     const double3 rj = ld3(g.r + 3 * j);
@@ -385,24 +388,24 @@ __global__ void __launch_bounds__(kRecvThreads) k_linker(Stat s, Geo g, Sites st
         }
     }
     if (terms & TERM_TAIL)  // tail-linker, energy owned by tails
-        for_near<kRecvThreads>(st.ch_t, s.n_tail, rj, 0.0, tid, nth, s_list, s_wsum, [&](int t) {
+        for_near<kLinkerThreads>(st.ch_t, s.n_tail, rj, 0.0, tid, nth, s_list, s_wsum, [&](int t) {
             dhev_recv(s.t_q[t] * s.q_l, s.c4pe, s.debye, s.k_ex, s.sig_tl, rj, ld3(g.r_t + 3 * t), F, Ed);
         });
     if (terms & TERM_LH) {
         // LH-DNA, energy owned by LH beads
-        for_near<kRecvThreads>(st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int i) {
+        for_near<kLinkerThreads>(st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int i) {
             int idx = i % s.nlh;
             idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
             dhev_recv(s.LH_q[idx] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[idx], rj, ld3(g.r_lh + 3 * i), F, Ed);
         });
         // LHG-DNA, energy on the core side
-        for_near<kRecvThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
+        for_near<kLinkerThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
             int m = x % s.n_lh_g;
             dhev_recv(s.LH_q[m] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[m], rj, ld3(st.G + 3 * x), F, Ed);
         });
     }
     double v[4] = {F.x, F.y, F.z, E};
-    block_sum_n<kRecvThreads, 4>(v, red);
+    block_sum_n<kLinkerThreads, 4>(v, red);
     if (tid == 0) {
         double *Fj = f.F + 3 * j;
         Fj[0] += v[0]; Fj[1] += v[1]; Fj[2] += v[2];
@@ -413,9 +416,9 @@ __global__ void __launch_bounds__(kRecvThreads) k_linker(Stat s, Geo g, Sites st
 // Tail bead (func.cpp:1793-2026 and the tail side of 2343-2476).  Assigns
 // tF[j].  Energy owner: bonds / angles starting at j, tail-tail pairs with
 // partner > j, tail-linker pairs, the anchor spring.
-__global__ void __launch_bounds__(kRecvThreads) k_tail(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
-    __shared__ double red[4 * (kRecvThreads / 64)];
-    __shared__ int s_list[kRecvThreads], s_wsum[kRecvThreads / 64];
+__global__ void __launch_bounds__(kTailThreads) k_tail(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
+    __shared__ double red[4 * (kTailThreads / 64)];
+    __shared__ int s_list[kTailThreads], s_wsum[kTailThreads / 64];
     const int j = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
     const int nt = s.n_tail;
     const int *tg = s.t_grp;
@@ -449,30 +452,30 @@ __global__ void __launch_bounds__(kRecvThreads) k_tail(Stat s, Geo g, Sites st, 
         }
         // tail-tail DH (1863-1891), skipping same-group neighbours
         const double qj = s.t_q[j];
-        for_near<kRecvThreads>(st.ch_t, nt, rj, 0.0, tid, nth, s_list, s_wsum, [&](int k) {
+        for_near<kTailThreads>(st.ch_t, nt, rj, 0.0, tid, nth, s_list, s_wsum, [&](int k) {
             if (k == j) return;
             if (!(tg[k] != tg[j] || abs(k - j) > 2)) return;
             dh_recv(qj * s.t_q[k], s.c4pe, s.debye, rj, ld3(g.r_t + 3 * k), F, (k > j) ? E : Ed);
         });
         // tail-linker (1898-1917)
-        for_near<kRecvThreads>(st.ch_l, s.n_lnk, rj, 0.0, tid, nth, s_list, s_wsum, [&](int pi) {
+        for_near<kTailThreads>(st.ch_l, s.n_lnk, rj, 0.0, tid, nth, s_list, s_wsum, [&](int pi) {
             dhev_recv(qj * s.q_l, s.c4pe, s.debye, s.k_ex, s.sig_tl, rj, ld3(g.r + 3 * s.lnk[pi]), F, E);
         });
     }
     if ((terms & TERM_LH) && s.t_fix[j] == 0) {
         // LH-tail and LHG-tail, tail side; anchored tails pass these to a core (k_core)
-        for_near<kRecvThreads>(st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int i) {
+        for_near<kTailThreads>(st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int i) {
             int idx = i % s.nlh;
             idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
             dhev_recv(s.LH_q[idx] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rj, ld3(g.r_lh + 3 * i), F, Ed);
         });
-        for_near<kRecvThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
+        for_near<kTailThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
             int m = x % s.n_lh_g;
             dhev_recv(s.LH_q[m] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], rj, ld3(st.G + 3 * x), F, Ed);
         });
     }
     double v[4] = {F.x, F.y, F.z, E};
-    block_sum_n<kRecvThreads, 4>(v, red);
+    block_sum_n<kTailThreads, 4>(v, red);
     if (tid == 0) {
         st3(f.tF + 3 * j, make_double3(v[0], v[1], v[2]));
         if (Epart) Epart[j] = v[3];
@@ -481,9 +484,9 @@ __global__ void __launch_bounds__(kRecvThreads) k_tail(Stat s, Geo g, Sites st, 
 
 // Free LH bead (func.cpp:2093-2781, LH side).  Assigns lF[j].  Energy
 // owner: bonds / angles at j, LH-LH pairs with partner > j, LH-DNA, LH-tail.
-__global__ void __launch_bounds__(kRecvThreads) k_lh(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
-    __shared__ double red[4 * (kRecvThreads / 64)];
-    __shared__ int s_list[kRecvThreads], s_wsum[kRecvThreads / 64];
+__global__ void __launch_bounds__(kLHThreads) k_lh(Stat s, Geo g, Sites st, Frc f, double *Epart, int terms) {
+    __shared__ double red[4 * (kLHThreads / 64)];
+    __shared__ int s_list[kLHThreads], s_wsum[kLHThreads / 64];
     const int j = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
     const int nlh = s.nlh;
     auto tpl = [&](int i) { int x = i % nlh; return (x < s.n_lh_n) ? x : x + s.n_lh_g; };
@@ -526,7 +529,7 @@ __global__ void __launch_bounds__(kRecvThreads) k_lh(Stat s, Geo g, Sites st, Fr
             }
         }
         // LH-LH (2190-2244)
-        for_near<kRecvThreads>(st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int k) {
+        for_near<kLHThreads>(st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int k) {
             if (k == j) return;
             int i2 = tpl(k);
             double sg = (s.LH_hh[idx] + s.LH_hh[i2]) / 2;
@@ -538,15 +541,15 @@ __global__ void __launch_bounds__(kRecvThreads) k_lh(Stat s, Geo g, Sites st, Fr
             }
         });
         // LH-DNA (2249-2279)
-        for_near<kRecvThreads>(st.ch_l, s.n_lnk, rj, 0.0, tid, nth, s_list, s_wsum, [&](int pi) {
+        for_near<kLHThreads>(st.ch_l, s.n_lnk, rj, 0.0, tid, nth, s_list, s_wsum, [&](int pi) {
             dhev_recv(s.LH_q[idx] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[idx], rj, ld3(g.r + 3 * s.lnk[pi]), F, E);
         });
         // LH-tail (2343-2398)
-        for_near<kRecvThreads>(st.ch_t, s.n_tail, rj, 0.0, tid, nth, s_list, s_wsum, [&](int t) {
+        for_near<kLHThreads>(st.ch_t, s.n_tail, rj, 0.0, tid, nth, s_list, s_wsum, [&](int t) {
             dhev_recv(s.LH_q[idx] * s.t_q[t], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rj, ld3(g.r_t + 3 * t), F, E);
         });
         // LHG of other cores acting on this bead (2590-2644), energy on the core side
-        for_near<kRecvThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
+        for_near<kLHThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
             int k = x / s.n_lh_g, m = x % s.n_lh_g;
             if (k == mine) return;
             dhev_recv(s.LH_q[m] * s.LH_q[idx], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[idx]) / 2, rj,
@@ -554,7 +557,7 @@ __global__ void __launch_bounds__(kRecvThreads) k_lh(Stat s, Geo g, Sites st, Fr
         });
     }
     double v[4] = {F.x, F.y, F.z, E};
-    block_sum_n<kRecvThreads, 4>(v, red);
+    block_sum_n<kLHThreads, 4>(v, red);
     if (tid == 0) {
         st3(f.lF + 3 * j, make_double3(v[0], v[1], v[2]));
         if (Epart) Epart[j] = v[3];
@@ -1064,8 +1067,8 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
         ql = fs->b[1];
         qc = fs->b[2];
     }
-    if (s.n_tail) { M(K_TAIL, 0); k_tail<<<s.n_tail, kRecvThreads, 0, qt>>>(s, g, st, f, E(off_tail), terms); M(K_TAIL, 1); }
-    if (s.n_LH) { M(K_LH, 0); k_lh<<<s.n_LH, kRecvThreads, 0, ql>>>(s, g, st, f, E(off_lh), terms); M(K_LH, 1); }
+    if (s.n_tail) { M(K_TAIL, 0); k_tail<<<s.n_tail, kTailThreads, 0, qt>>>(s, g, st, f, E(off_tail), terms); M(K_TAIL, 1); }
+    if (s.n_LH) { M(K_LH, 0); k_lh<<<s.n_LH, kLHThreads, 0, ql>>>(s, g, st, f, E(off_lh), terms); M(K_LH, 1); }
     if (terms & TERM_ELEC) {
         long long nn = (long long)s.n_c * s.n_c;
         M(K_CCLIST, 0);
@@ -1086,7 +1089,7 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
         (void)hipEventRecord(fs->mech, q);
         (void)hipStreamWaitEvent(ql, fs->mech, 0);
     }
-    if (s.n_lnk) { M(K_LINKER, 0); k_linker<<<s.n_lnk, kRecvThreads, 0, ql>>>(s, g, st, f, E(off_lnk), terms); M(K_LINKER, 1); }
+    if (s.n_lnk) { M(K_LINKER, 0); k_linker<<<s.n_lnk, kLinkerThreads, 0, ql>>>(s, g, st, f, E(off_lnk), terms); M(K_LINKER, 1); }
     M(K_CORE, 0);
     const int *cs = w.core_split;
     k_core<<<dim3(s.n_c, cs[0] + cs[1] + cs[2]), kCoreThreads, 0, q>>>(s, g, st, f, w.core_part, w.core_cnt,
