@@ -206,6 +206,101 @@ __device__ __forceinline__ void acc_to_lds(double *C, V acc[2][2], double sign, 
             }
 }
 
+// Copies up to four tiles from global memory into LDS with every global
+// load of the thread in flight before the first LDS store (n: tiles used).
+__device__ __forceinline__ void load_tiles(int n, const double *const (&src)[4], const int (&ld)[4],
+                                           double *const (&dst)[4]) {
+    constexpr int PER = TB * TB / 2 / NTHR;
+    dbl2 v[4][PER];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+        if (t < n)
+#pragma unroll
+            for (int i = 0; i < PER; i++) {
+                const int e = threadIdx.x + i * NTHR, r = e / (TB / 2), c = 2 * (e % (TB / 2));
+                v[t][i] = *reinterpret_cast<const dbl2 *>(src[t] + (size_t)r * ld[t] + c);
+            }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+        if (t < n)
+#pragma unroll
+            for (int i = 0; i < PER; i++) {
+                const int e = threadIdx.x + i * NTHR, r = e / (TB / 2), c = 2 * (e % (TB / 2));
+                dst[t][r * LDP + c] = v[t][i].x;
+                dst[t][r * LDP + c + 1] = v[t][i].y;
+            }
+}
+
+// 16x16 output blocks on the FP64 matrix cores by one wavefront (mapping of
+// mma_f64: register v of lane l is element (l/16 + 4v, l%16) of the block).
+// acc += A[16I.., 0:K) B[16J.., 0:K)^T
+__device__ __forceinline__ dbl4 blk_mma(const double *A, const double *B, int I, int J, int K, dbl4 acc) {
+    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
+    for (int kk = 0; kk < K; kk += 4) {
+        const double a = A[(16 * I + i16) * LDP + kk + k4];
+        const double b = B[(16 * J + i16) * LDP + kk + k4];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+__device__ __forceinline__ void blk_to_lds(double *C, int I, int J, dbl4 acc, double sign, bool overwrite) {
+    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        double *c = C + (16 * I + k4 + 4 * v) * LDP + 16 * J + i16;
+        if (overwrite) *c = sign * acc[v];
+        else *c += sign * acc[v];
+    }
+}
+
+// Panel product P = A inv(L)^T with inv(L) lower triangular (zero above
+// its diagonal 16x16 blocks): output block column J needs k < 16 (J + 1)
+// only, and wavefront w takes the 16-row stripe w of P, so every
+// wavefront issues the same 40 products.  Skipping the zero blocks adds
+// nothing to any sum, so P is the full product bit for bit.
+__device__ __forceinline__ void mma_panel(const double *As, const double *Li, dbl4 (&acc)[4]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < TB; kk += 4) {
+        const double a = As[(16 * w + i16) * LDP + kk + k4];
+#pragma unroll
+        for (int J = 0; J < 4; J++)
+            if (kk < 16 * (J + 1)) {
+                const double b = Li[(16 * J + i16) * LDP + kk + k4];
+                acc[J] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[J], 0, 0, 0);
+            }
+    }
+}
+__device__ __forceinline__ void panel_to_lds(double *C, const dbl4 (&acc)[4]) {
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int J = 0; J < 4; J++) blk_to_lds(C, w, J, acc[J], 1.0, true);
+}
+
+// Diagonal tile update C -= P P^T on its ten lower 16x16 blocks only (the
+// factor never reads above the diagonal), three or two per wavefront.
+__device__ __forceinline__ void syrk_lower(const double *P, double *C) {
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // (I, J) blocks of wavefront w: {00 10 11} {20 21 22} {30 31} {32 33}
+    const int I0 = w < 2 ? 2 * w : 3, J0 = w < 2 ? 0 : 2 * (w - 2);
+    const dbl4 z = {0.0, 0.0, 0.0, 0.0};
+    if (w == 0) {
+        const dbl4 c0 = blk_mma(P, P, 0, 0, TB, z), c1 = blk_mma(P, P, 1, 0, TB, z), c2 = blk_mma(P, P, 1, 1, TB, z);
+        blk_to_lds(C, 0, 0, c0, -1.0, false);
+        blk_to_lds(C, 1, 0, c1, -1.0, false);
+        blk_to_lds(C, 1, 1, c2, -1.0, false);
+    } else if (w == 1) {
+        const dbl4 c0 = blk_mma(P, P, 2, 0, TB, z), c1 = blk_mma(P, P, 2, 1, TB, z), c2 = blk_mma(P, P, 2, 2, TB, z);
+        blk_to_lds(C, 2, 0, c0, -1.0, false);
+        blk_to_lds(C, 2, 1, c1, -1.0, false);
+        blk_to_lds(C, 2, 2, c2, -1.0, false);
+    } else {
+        const dbl4 c0 = blk_mma(P, P, I0, J0, TB, z), c1 = blk_mma(P, P, I0, J0 + 1, TB, z);
+        blk_to_lds(C, I0, J0, c0, -1.0, false);
+        blk_to_lds(C, I0, J0 + 1, c1, -1.0, false);
+    }
+}
+
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
     int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
     int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
@@ -507,51 +602,53 @@ __global__ void __launch_bounds__(256) k_potrf0(double *A, int ld, double *Linv,
 // exchange for one launch (and one kernel boundary) per block column.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, double *Linv, int *neg) {
-    __shared__ double W[TB * LDP], As[TB * LDP], Bs[TB * LDP];
+    __shared__ double W[TB * LDP], As[TB * LDP], Bs[TB * LDP], Cs[TB * LDP];
     int ii, jj;
     tri_index(blockIdx.x, ii, jj);
     const int ib = kb + 1 + ii, jb = kb + 1 + jj;
-    load_tile(Linv + (size_t)kb * TB * TB, TB, 0, 0, W);
-    load_tile(A, ld, ib * TB, kb * TB, As);
-    if (ib != jb) load_tile(A, ld, jb * TB, kb * TB, Bs);
-    __syncthreads();
+    const bool diag = ib == jb;
     {
-        dbl4 pi[2][2], pj[2][2];
-        for (int a = 0; a < 2; a++)
-            for (int b = 0; b < 2; b++) pi[a][b] = pj[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-        mma_f64(As, W, pi);
-        if (ib != jb) mma_f64(Bs, W, pj);
-        __syncthreads();
-        acc_to_lds(As, pi, 1.0, true);
-        if (ib != jb) acc_to_lds(Bs, pj, 1.0, true);
+        // inv(L_kb,kb), the panel blocks and the target tile, all in flight
+        const double *const src[4] = {Linv + (size_t)kb * TB * TB, A + (size_t)ib * TB * ld + kb * TB,
+                                      A + (size_t)ib * TB * ld + jb * TB, A + (size_t)jb * TB * ld + kb * TB};
+        const int lds[4] = {TB, ld, ld, ld};
+        double *const dst[4] = {W, As, Cs, Bs};
+        load_tiles(diag ? 3 : 4, src, lds, dst);
     }
     __syncthreads();
-    if (ib == jb) store_tile(A, ld, kb * TB, ib * TB, As, false);
-    const double *Bp = (ib != jb) ? Bs : As;
+    {
+        dbl4 pi[4], pj[4];
+        for (int J = 0; J < 4; J++) pi[J] = pj[J] = dbl4{0.0, 0.0, 0.0, 0.0};
+        mma_panel(As, W, pi);
+        if (!diag) mma_panel(Bs, W, pj);
+        __syncthreads();
+        panel_to_lds(As, pi);
+        if (!diag) panel_to_lds(Bs, pj);
+    }
+    __syncthreads();
+    if (diag) store_tile(A, ld, kb * TB, ib * TB, As, false);
     if (MODE == 0) {
-        dbl4 acc[2][2];
-        for (int a = 0; a < 2; a++)
-            for (int b = 0; b < 2; b++) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-        mma_f64(As, Bp, acc);
-        __syncthreads();
-        load_tile(A, ld, ib * TB, jb * TB, W);
-        __syncthreads();
-        acc_to_lds(W, acc, -1.0, false);
+        if (diag) {
+            syrk_lower(As, Cs);
+        } else {
+            dbl4 acc[2][2];
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+            mma_f64(As, Bs, acc);
+            acc_to_lds(Cs, acc, -1.0, false);
+        }
     } else {
         flt4 acc[2][2];
         for (int a = 0; a < 2; a++)
             for (int b = 0; b < 2; b++) acc[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
-        mma_f32(As, Bp, acc);
-        __syncthreads();
-        load_tile(A, ld, ib * TB, jb * TB, W);
-        __syncthreads();
-        acc_to_lds(W, acc, -1.0, false);
+        mma_f32(As, diag ? As : Bs, acc);
+        acc_to_lds(Cs, acc, -1.0, false);
     }
     __syncthreads();
     if (ii == 0 && jj == 0)
-        diag_factor_inv(W, As, Bs, A, ld, ib * TB, Linv + (size_t)ib * TB * TB, neg);
+        diag_factor_inv(Cs, W, As, A, ld, ib * TB, Linv + (size_t)ib * TB * TB, neg);
     else
-        store_tile(A, ld, ib * TB, jb * TB, W, ib == jb);
+        store_tile(A, ld, ib * TB, jb * TB, Cs, diag);
 }
 
 // rd = s2dt * L * p (rd_cal, func_cuda.cu:634-647): one wavefront per row,

@@ -868,18 +868,21 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
 
 // The core-core charge sums of core kc: its row (as c1) and column (as c2)
 // of the slot matrix that k_cc_work fills (func.cpp:1489-1566), one
-// wavefront per core, lanes over the (other core, chunk) slots in index
-// order and a fixed shuffle tree; adds to F, T of the core bead and owns
+// workgroup per core, threads over the (other core, unit) slots in index
+// order and a fixed-order block sum; adds to F, T of the core bead and owns
 // the energy of the pairs where kc is c1.
-__global__ void __launch_bounds__(64) k_cc_gather(Stat s, Geo g, Frc f, const unsigned char *cc_valid, int nchunk,
-                                                   const double *cc_slots, double *Ecc, int terms) {
+constexpr int kGatherThreads = 256;
+__global__ void __launch_bounds__(kGatherThreads) k_cc_gather(Stat s, Geo g, Frc f, const unsigned char *cc_valid,
+                                                              int nchunk, const double *cc_slots, double *Ecc,
+                                                              int terms) {
+    __shared__ double red[10 * (kGatherThreads / 64)];
     const int kc = blockIdx.x, lane = threadIdx.x;
     const int cb = s.core_bead[kc];
     double3 F = zero3(), T = zero3(), Tw = zero3();
     double Ec = 0.0;
     // without the electrostatic terms only the energy slot is cleared
     const int nx = (terms & TERM_ELEC) ? s.n_c * nchunk : 0;
-    for (int x = lane; x < nx; x += 64) {
+    for (int x = lane; x < nx; x += kGatherThreads) {
         const int o = x / nchunk, ch = x % nchunk;
         if (o == kc) continue;
         const bool first = kc < o;
@@ -896,9 +899,7 @@ __global__ void __launch_bounds__(64) k_cc_gather(Stat s, Geo g, Frc f, const un
         }
     }
     double v[10] = {F.x, F.y, F.z, T.x, T.y, T.z, Tw.x, Tw.y, Tw.z, Ec};
-#pragma unroll
-    for (int q = 0; q < 10; q++)
-        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
+    block_sum_n<kGatherThreads, 10>(v, red);
     if (lane == 0) {
         const double3 ac = ld3(g.a + 3 * cb), bc = ld3(g.b + 3 * cb), cc = ld3(g.c + 3 * cb);
         const double3 tw = make_double3(v[6], v[7], v[8]);
@@ -1102,7 +1103,7 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
         }
     }
     M(K_CCGATHER, 0);
-    k_cc_gather<<<s.n_c, 64, 0, q>>>(s, g, f, w.cc_valid, w.cc_nkb * w.cc_nlc, w.cc_slots, E(off_cc), terms);
+    k_cc_gather<<<s.n_c, kGatherThreads, 0, q>>>(s, g, f, w.cc_valid, w.cc_nkb * w.cc_nlc, w.cc_slots, E(off_cc), terms);
     M(K_CCGATHER, 1);
 }
 
