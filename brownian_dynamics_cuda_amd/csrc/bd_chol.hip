@@ -12,7 +12,9 @@
 // tiles it needs as A_ib,kb inv(L_kb,kb)^T (FP64 matrix cores), updates its
 // trailing tile A_ib,jb -= L_ib,kb L_jb,kb^T (FP64 or FP32 matrix cores),
 // and the workgroup owning tile (kb+1,kb+1) then factors it and inverts
-// the factor (two wavefronts, FP64, the inverse one panel behind).
+// the factor (four wavefronts, FP64, the inverse built beside the pivots).
+// The solved off-diagonal tile L_ib,kb stays parked in the upper block
+// (kb, ib) of the same storage.
 #include "bd_dev.h"
 #include "bd_kernels.h"
 
@@ -485,9 +487,21 @@ __device__ __forceinline__ void s3_block(const double *LP, const double *X, doub
 // as soon as their columns are final, X_10 (step 5), X_21, X_20 and
 // S_32 = L_32 X_22 (step 7); S: the wavefront's 16x17 scratch, S3: the
 // three partial sums S_3j.
+// This lane's 16 columns of row d0 + lane of L to global memory (zeros above
+// the diagonal: a lane above a pivot column takes 0 at that pivot).
+__device__ __forceinline__ void store_lrow(const double (&g)[16], double *row) {
+#pragma unroll
+    for (int c = 0; c < 16; c += 2) {
+        dbl2 v;
+        v.x = g[c];
+        v.y = g[c + 1];
+        *reinterpret_cast<dbl2 *>(row + c) = v;
+    }
+}
+
 template <int P>
 __device__ __forceinline__ void factor_step(double (&g)[16], double *LP, double *ivs, double *X, double *S,
-                                            double *S3, double &myinv, int &nneg, int w) {
+                                            double *S3, double &myinv, int &nneg, int w, double *lrow) {
     constexpr int O = P >> 1, LOC = P & 1;
     if (w == O) {
         if constexpr (LOC == 0 && P > 0) {
@@ -497,6 +511,9 @@ __device__ __forceinline__ void factor_step(double (&g)[16], double *LP, double 
             panel_update16<0, 16>(g, x, LP + (P - 1) * (TB * 8), 16 * w);
         }
         pivot_panel<P, 8 * LOC>(g, LP, ivs, myinv, nneg);
+        // the owner's second panel completes its columns of L: stored now,
+        // beside the steps that follow
+        if constexpr (LOC == 1) store_lrow(g, lrow);
         if constexpr (LOC == 0) {
             double x[8];
 #pragma unroll
@@ -535,47 +552,42 @@ __device__ __forceinline__ void diag_factor_inv(double *T, double *LP, double *X
     for (int c = 0; c < 16; c++) g[c] = T[lane * LDP + 16 * w + c];
     __syncthreads();
     double *ivs = T, *S = T + TB + w * (16 * 17), *S3 = T + TB + 4 * (16 * 17);
+    double *lrow = A + (size_t)(d0 + lane) * ld + d0 + 16 * w;
     int nneg = 0;
     double myinv = 0.0;
-    factor_step<0>(g, LP, ivs, X, S, S3, myinv, nneg, w);
-    factor_step<1>(g, LP, ivs, X, S, S3, myinv, nneg, w);
-    factor_step<2>(g, LP, ivs, X, S, S3, myinv, nneg, w);
-    factor_step<3>(g, LP, ivs, X, S, S3, myinv, nneg, w);
-    factor_step<4>(g, LP, ivs, X, S, S3, myinv, nneg, w);
-    factor_step<5>(g, LP, ivs, X, S, S3, myinv, nneg, w);
-    factor_step<6>(g, LP, ivs, X, S, S3, myinv, nneg, w);
-    factor_step<7>(g, LP, ivs, X, S, S3, myinv, nneg, w);
-    // L: each wavefront its 16 columns (zeros above the diagonal: a lane
-    // above a pivot column takes 0 at that pivot)
-    double *row = A + (size_t)(d0 + lane) * ld + d0 + 16 * w;
-#pragma unroll
-    for (int c = 0; c < 16; c += 2) {
-        dbl2 v;
-        v.x = g[c];
-        v.y = g[c + 1];
-        *reinterpret_cast<dbl2 *>(row + c) = v;
-    }
+    factor_step<0>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
+    factor_step<1>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
+    factor_step<2>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
+    factor_step<3>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
+    factor_step<4>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
+    factor_step<5>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
+    factor_step<6>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
+    factor_step<7>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
     if (nneg && lane == 0) atomicAdd(neg, nneg);
-    // X_33 beside the partial sums S_30, S_31; then X_3j = -X_33 S_3j
+    // rows 0..47 of inv(L) are final: wavefronts 0-2 store their stripe
+    // (zeros in the blocks above the diagonal) beside X_33 and S_30, S_31
+    auto store_xrows = [&](int r0, int nr, int tid, int nthr) {
+        for (int e = tid; e < nr * (TB / 2); e += nthr) {
+            const int r = r0 + e / (TB / 2), c = 2 * (e % (TB / 2));
+            dbl2 v;
+            const bool keep = (r >> 4) >= (c >> 4);
+            v.x = keep ? X[r * LDP + c] : 0.0;
+            v.y = keep ? X[r * LDP + c + 1] : 0.0;
+            *reinterpret_cast<dbl2 *>(Linv + r * TB + c) = v;
+        }
+    };
     if (w == 3) diag_block_inverse(LP, ivs, X, 3);
     else if (w == 0) s3_block(LP, X, S3, 0);
     else if (w == 1) s3_block(LP, X, S3 + 16 * 17, 1);
+    if (w < 3) store_xrows(16 * w, 16, lane, 64);
     __syncthreads();
+    // X_3j = -X_33 S_3j, then the last stripe of inv(L)
     if (w < 3) {
         const dbl4 z = {0.0, 0.0, 0.0, 0.0};
         put_block(X + 48 * LDP + 16 * w, LDP, mm_xs(z, X, S3 + w * (16 * 17), 3), -1.0);
     }
     __syncthreads();
-    // inv(L), zeros in the blocks above the diagonal
-#pragma unroll
-    for (int i = 0; i < TB * TB / 2 / NTHR; i++) {
-        const int e = threadIdx.x + i * NTHR, r = e / (TB / 2), c = 2 * (e % (TB / 2));
-        dbl2 v;
-        const bool keep = (r >> 4) >= (c >> 4);
-        v.x = keep ? X[r * LDP + c] : 0.0;
-        v.y = keep ? X[r * LDP + c + 1] : 0.0;
-        *reinterpret_cast<dbl2 *>(Linv + r * TB + c) = v;
-    }
+    store_xrows(48, 16, threadIdx.x, NTHR);
 }
 
 // first diagonal tile
@@ -593,7 +605,8 @@ __global__ void __launch_bounds__(256) k_potrf0(double *A, int ld, double *Linv,
 //      Linv; the workgroup of the diagonal tile (ib, ib) stores L_ib,kb, so
 //      every panel tile is written exactly once, into the upper block
 //      (kb, ib), because other workgroups of this launch still read
-//      A_ib,kb (k_panel_copy moves them after the last step);
+//      A_ib,kb; the tiles stay there (k_noise and the host copy of the
+//      factor read them from there);
 //   2. A_ib,jb -= L_ib,kb L_jb,kb^T on the matrix cores (FP64, or FP32 with
 //      FP32 accumulation when MODE != 0);
 //   3. the workgroup of tile (kb+1, kb+1) factors and inverts it, which the
@@ -652,8 +665,11 @@ __global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, double 
 }
 
 // rd = s2dt * L * p (rd_cal, func_cuda.cu:634-647): one wavefront per row,
-// coalesced over the row; rows outside the hydrodynamic set use their
-// self-mobility: s2dt * sqrt(a1 / rad) * p.
+// lane k % 64 over the row; rows outside the hydrodynamic set use their
+// self-mobility: s2dt * sqrt(a1 / rad) * p.  L is read where the factor
+// leaves it: the diagonal tile of the row in place, the off-diagonal tile
+// (ib, t) parked in the upper block (t, ib) (a row of the parked tile is
+// contiguous, so every read is one 512-byte segment).
 __global__ void __launch_bounds__(256) k_noise(const double *A, int ld, int m, int nD3, const double *p,
                                                const double *rad_all, double s2dt, double a1, double *rd) {
     const int lane = threadIdx.x & 63;
@@ -663,9 +679,10 @@ __global__ void __launch_bounds__(256) k_noise(const double *A, int ld, int m, i
         if (lane == 0) rd[row] = s2dt * sqrt(a1 / rad_all[row / 3]) * p[row];
         return;
     }
-    const double *Lr = A + (size_t)row * ld;
+    const int ib = row / TB, r = row % TB;
     double v = 0.0;
-    for (int k = lane; k <= row; k += 64) v += Lr[k] * p[k];
+    for (int t = 0; t < ib; t++) v += A[(size_t)(t * TB + r) * ld + ib * TB + lane] * p[t * TB + lane];
+    if (lane <= r) v += A[(size_t)row * ld + ib * TB + lane] * p[ib * TB + lane];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     if (lane == 0) rd[row] = s2dt * v;
 }
@@ -675,24 +692,6 @@ void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, co
     const int nt = c.mpad / TB;
     k_rpy<<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2);
     (void)n_hi;
-}
-
-// Moves every solved panel tile from its upper block (kb, ib) to (ib, kb).
-__global__ void __launch_bounds__(256) k_panel_copy(double *A, int ld) {
-    int ii, jj;
-    tri_index(blockIdx.x, ii, jj);
-    const int ib = ii + 1, kb = jj;
-    dbl2 v[TB * TB / 2 / NTHR];
-#pragma unroll
-    for (int i = 0; i < TB * TB / 2 / NTHR; i++) {
-        const int e = threadIdx.x + i * NTHR, r = e / (TB / 2), c = 2 * (e % (TB / 2));
-        v[i] = *reinterpret_cast<const dbl2 *>(A + (size_t)(kb * TB + r) * ld + ib * TB + c);
-    }
-#pragma unroll
-    for (int i = 0; i < TB * TB / 2 / NTHR; i++) {
-        const int e = threadIdx.x + i * NTHR, r = e / (TB / 2), c = 2 * (e % (TB / 2));
-        *reinterpret_cast<dbl2 *>(A + (size_t)(ib * TB + r) * ld + kb * TB + c) = v[i];
-    }
 }
 
 void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_arg, hipEvent_t gate, int gate_kb) {
@@ -713,11 +712,6 @@ void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_a
         if (gate && kb == gate_kb) (void)hipEventRecord(gate, q);
     }
     if (gate && gate_kb >= nt - 1) (void)hipEventRecord(gate, q);
-    if (nt > 1) {
-        M_(K_PANEL_COPY, 0);
-        k_panel_copy<<<nt * (nt - 1) / 2, 256, 0, q>>>(c.A, c.ld);
-        M_(K_PANEL_COPY, 1);
-    }
 }
 
 void launch_noise(const Stat &s, int n_hi, int n_D3, const CholWork &c, const double *p, const double *rad_all,

@@ -55,7 +55,7 @@ int fail(int code, const char *fmt, ...) {
 
 const char *kKernelNames[K_COUNT] = {
     "sites", "mech", "linker", "tail", "lh", "core", "cc_list", "cc_work", "rpy",
-    "potrf", "panel_copy", "chol_step", "noise", "frames", "betas", "rotate", "translate", "average",
+    "potrf", "chol_step", "noise", "frames", "betas", "rotate", "translate", "average",
     "commit", "energy_reduce", "cc_gather"};
 
 }  // namespace
@@ -802,12 +802,21 @@ int bd_get_factor(bd_context *c, double *L, int *neg) {
     if (check(c)) return BD_E_ARG;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
-    const int m = c->chol.m;
+    const int m = c->chol.m, mp = c->chol.mpad;
     if (L) {
-        HIPCHK(hipMemcpy2D(L, sizeof(double) * m, c->chol.A, sizeof(double) * c->chol.ld, sizeof(double) * m, m,
-                           hipMemcpyDeviceToHost));
+        // the factor's storage: diagonal tiles in place, the off-diagonal
+        // tile (ib, kb) parked in the upper block (kb, ib)
+        std::vector<double> raw((size_t)mp * mp);
+        HIPCHK(hipMemcpy2D(raw.data(), sizeof(double) * mp, c->chol.A, sizeof(double) * c->chol.ld,
+                           sizeof(double) * mp, mp, hipMemcpyDeviceToHost));
         for (int i = 0; i < m; i++)
-            for (int j = i + 1; j < m; j++) L[(size_t)i * m + j] = 0.0;
+            for (int j = 0; j < m; j++) {
+                const int ib = i / 64, kb = j / 64;
+                double v = 0.0;
+                if (kb < ib) v = raw[(size_t)(kb * 64 + i % 64) * mp + ib * 64 + j % 64];
+                else if (j <= i) v = raw[(size_t)i * mp + j];
+                L[(size_t)i * m + j] = v;
+            }
     }
     if (neg) HIPCHK(hipMemcpy(neg, c->chol.neg, sizeof(int), hipMemcpyDeviceToHost));
     return BD_OK;

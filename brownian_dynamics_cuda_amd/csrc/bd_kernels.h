@@ -25,7 +25,7 @@ void launch_sites(const Stat &s, const Geo &g, const Sites &st, hipStream_t q);
 // mark(arg, id, phase): optional timing hook called around each kernel
 // (phase 0 before, 1 after); ids are the K_* values below.
 enum KernelId { K_SITES, K_MECH, K_LINKER, K_TAIL, K_LH, K_CORE, K_CCLIST, K_CCWORK,
-                K_RPY, K_POTRF, K_PANEL_COPY, K_CHOL_STEP, K_NOISE, K_FRAMES, K_BETAS, K_ROTATE, K_TRANSLATE,
+                K_RPY, K_POTRF, K_CHOL_STEP, K_NOISE, K_FRAMES, K_BETAS, K_ROTATE, K_TRANSLATE,
                 K_AVG, K_COMMIT, K_EREDUCE, K_CCGATHER, K_COUNT };
 typedef void (*MarkFn)(void *, int, int);
 // Extra streams of a force pass: after the charge sites, the tail, the
