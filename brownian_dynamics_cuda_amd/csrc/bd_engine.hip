@@ -215,7 +215,7 @@ void enqueue_step(bd_context *c, bool refactor) {
     // the early block columns fill the GPU, and force kernels beside them
     // hold the CUs that the diagonal-tile workgroups wait for
     const int nt = c->chol.mpad / 64;
-    const int gate_kb = c->gate_kb >= 0 ? c->gate_kb : (nt - 1) * 45 / 100;
+    const int gate_kb = c->gate_kb >= 0 ? c->gate_kb : (nt - 1) * 55 / 100;
     const bool gated = refactor && qf != q;
     if (refactor) enqueue_factor(c, c->cur, qf, gated ? c->ev_gate : nullptr, gate_kb);
     M(K_NOISE, 0); launch_noise(s, c->n_hi, c->n_D3, c->chol, c->p, c->rad_all, c->rd, qf); M(K_NOISE, 1);

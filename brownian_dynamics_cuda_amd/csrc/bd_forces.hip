@@ -367,24 +367,35 @@ __global__ void __launch_bounds__(kLinkerThreads) k_linker(Stat s, Geo g, Sites 
             F += (g1 / dist) * dv;
             if (i > j) E += Ep;
         }
-        // DNA-core: every charge of the eligible cores within 25 nm
-        // (the core test is uniform across the workgroup); energy on the core side
-        for (int k = 0; k < s.n_c; k++) {
-            int cb = s.core_bead[k];
-            if (!s.elig[cb]) continue;
-            double3 dc = ld3(g.r + 3 * cb) - rj;
-            if (sqrt(dot(dc, dc)) > 25.0) continue;
-            const bool dh = abs(cb - j) > 1;
-            const double *Q = st.Q + 3 * (size_t)k * s.Nq;
-            for (int q = tid; q < s.Nq; q += nth) {
-                double3 z = rj - ld3(Q + 3 * q);
-                double d = sqrt(dot(z, z));
-                double cq = s.cq[q];
-                double g1 = 0.0;
-                if (dh) g1 += dh_core(s.k_e * s.q_l * cq, s.debye, d, Ed);
-                if (d <= 8.0 && cq > 0) g1 += ev_core(s.k_ex, s.sig_dc, d, Ed);
-                F += (g1 / d) * z;
+        // DNA-core: every charge of the eligible cores within 25 nm, the
+        // cores tested by the whole workgroup and listed in index order;
+        // energy on the core side
+        for (int base = 0; base < s.n_c; base += nth) {
+            const int k = base + tid;
+            bool keep = false;
+            if (k < s.n_c) {
+                const int cb = s.core_bead[k];
+                if (s.elig[cb]) {
+                    const double3 dc = ld3(g.r + 3 * cb) - rj;
+                    keep = !(sqrt(dot(dc, dc)) > 25.0);
+                }
             }
+            const int cnt = compact_ordered<kLinkerThreads>(keep, k, s_list, s_wsum);
+            for (int e = 0; e < cnt; e++) {
+                const int kc = s_list[e];
+                const bool dh = abs(s.core_bead[kc] - j) > 1;
+                const double *Q = st.Q + 3 * (size_t)kc * s.Nq;
+                for (int q = tid; q < s.Nq; q += nth) {
+                    double3 z = rj - ld3(Q + 3 * q);
+                    double d = sqrt(dot(z, z));
+                    double cq = s.cq[q];
+                    double g1 = 0.0;
+                    if (dh) g1 += dh_core(s.k_e * s.q_l * cq, s.debye, d, Ed);
+                    if (d <= 8.0 && cq > 0) g1 += ev_core(s.k_ex, s.sig_dc, d, Ed);
+                    F += (g1 / d) * z;
+                }
+            }
+            __syncthreads();
         }
     }
     if (terms & TERM_TAIL)  // tail-linker, energy owned by tails
