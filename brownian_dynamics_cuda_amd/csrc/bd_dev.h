@@ -207,20 +207,22 @@ __device__ __forceinline__ bool dhev_recv(double q1q2, double c4pe, double kappa
     if (!(d2 < 49.01)) return false;
     double d = sqrt(d2);
     if (!(d < 7.0)) return false;  // beyond both cut-offs: exactly zero
-    double fs = 0.0;
-    double e = exp(-kappa * d);
-    fs = ((q1q2 * (kappa * d + 1)) / (c4pe * d * d)) * e;
-    E += (q1q2 / (c4pe * d)) * e;
+    // the reference's quotients by d, d^2, d^7 and d^13 as products with
+    // one reciprocal of d
+    const double id = 1.0 / d, qe = q1q2 / c4pe * exp_neg(-kappa * d);
+    double fs = qe * (kappa * d + 1) * (id * id);
+    E += qe * id;
     if (d < 4.0) {
-        double x = sigma / d, x2 = x * x, x6 = x2 * x2 * x2;
-        double s6 = sigma * sigma, s6b = s6 * s6 * s6;
-        double d7 = d * d2 * d2 * d2;  // d^7
-        fs += -kev * ((6 * s6b) / d7 - (12 * (s6b * s6b)) / (d7 * d2 * d2 * d2));
+        const double x = sigma * id, x2 = x * x, x6 = x2 * x2 * x2;
+        const double s6 = sigma * sigma, s6b = s6 * s6 * s6;
+        const double i2 = id * id, i7 = i2 * i2 * i2 * id;  // d^-7
+        fs += -kev * (6 * s6b * i7 - 12 * (s6b * s6b) * (i7 * i2 * i2 * i2));
         E += kev * (x6 * x6 - x6);
     }
-    f.x += clamp09(fs * dv.x / d);
-    f.y += clamp09(fs * dv.y / d);
-    f.z += clamp09(fs * dv.z / d);
+    const double fd = fs * id;
+    f.x += clamp09(fd * dv.x);
+    f.y += clamp09(fd * dv.y);
+    f.z += clamp09(fd * dv.z);
     return true;
 }
 
@@ -232,12 +234,12 @@ __device__ __forceinline__ bool dh_recv(double q1q2, double c4pe, double kappa, 
     if (!(d2 < 49.01)) return false;
     double d = sqrt(d2);
     if (!(d < 7.0)) return false;
-    double e = exp(-kappa * d);
-    double fs = ((q1q2 * (kappa * d + 1)) / (c4pe * d * d)) * e;
-    E += (q1q2 / (c4pe * d)) * e;
-    f.x += clamp09(fs * dv.x / d);
-    f.y += clamp09(fs * dv.y / d);
-    f.z += clamp09(fs * dv.z / d);
+    const double id = 1.0 / d, qe = q1q2 / c4pe * exp_neg(-kappa * d);
+    const double fd = qe * (kappa * d + 1) * (id * id * id);
+    E += qe * id;
+    f.x += clamp09(fd * dv.x);
+    f.y += clamp09(fd * dv.y);
+    f.z += clamp09(fd * dv.z);
     return true;
 }
 
@@ -249,14 +251,16 @@ __device__ __forceinline__ bool ev_recv(double kev, double sigma, double3 p, dou
     if (!(d2 < 16.01)) return false;
     double d = sqrt(d2);
     if (!(d < 4.0)) return false;
-    double x = sigma / d, x2 = x * x, x6 = x2 * x2 * x2;
-    double s6 = sigma * sigma, s6b = s6 * s6 * s6;
-    double d7 = d * d2 * d2 * d2;
-    double fs = -kev * ((6 * s6b) / d7 - (12 * (s6b * s6b)) / (d7 * d2 * d2 * d2));
+    const double id = 1.0 / d;
+    const double x = sigma * id, x2 = x * x, x6 = x2 * x2 * x2;
+    const double s6 = sigma * sigma, s6b = s6 * s6 * s6;
+    const double i2 = id * id, i7 = i2 * i2 * i2 * id;  // d^-7
+    const double fs = -kev * (6 * s6b * i7 - 12 * (s6b * s6b) * (i7 * i2 * i2 * i2));
     E += kev * (x6 * x6 - x6);
-    f.x += clamp09(fs * dv.x / d);
-    f.y += clamp09(fs * dv.y / d);
-    f.z += clamp09(fs * dv.z / d);
+    const double fd = fs * id;
+    f.x += clamp09(fd * dv.x);
+    f.y += clamp09(fd * dv.y);
+    f.z += clamp09(fd * dv.z);
     return true;
 }
 

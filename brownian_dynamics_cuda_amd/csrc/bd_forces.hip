@@ -321,9 +321,9 @@ __global__ void k_mech(Stat s, Geo g, Frc f, double *Epart, int terms) {
 // ---------------------------------------------------------------------------
 // scalar g1 of the reference for a charge pair at distance d; adds energy
 __device__ __forceinline__ double dh_core(double ke_q1q2, double kappa, double d, double &E) {
-    double e = exp(-kappa * d);
-    E += (ke_q1q2 / d) * e;
-    return ke_q1q2 * e * (kappa * d + 1.0) / (d * d);
+    const double id = 1.0 / d, qe = ke_q1q2 * exp_neg(-kappa * d);
+    E += qe * id;
+    return qe * (kappa * d + 1.0) * (id * id);
 }
 __device__ __forceinline__ double ev_core(double kex, double sg, double d, double &E) {
     double x = sg / d, x2 = x * x, x6 = x2 * x2 * x2, x7 = x6 * x, x13 = x6 * x7;
@@ -361,8 +361,9 @@ __global__ void __launch_bounds__(kLinkerThreads) k_linker(Stat s, Geo g, Sites 
             double dist = sqrt(dot(dv, dv));
             if (dist > 25.0) continue;
             double Ep = 0.0;
-            double g1 = s.k_e * qlql * exp(-kll * dist) * (kll * dist + 1.0) / (dist * dist);
-            Ep += (s.k_e * qlql / dist) * exp(-kll * dist);
+            const double id = 1.0 / dist, qe = s.k_e * qlql * exp_neg(-kll * dist);
+            double g1 = qe * (kll * dist + 1.0) * (id * id);
+            Ep += qe * id;
             if (dist <= 8) g1 += ev_core(s.k_ex, s.sig_dd, dist, Ep);
             F += (g1 / dist) * dv;
             if (i > j) E += Ep;
