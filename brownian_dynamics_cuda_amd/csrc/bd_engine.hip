@@ -238,8 +238,7 @@ void enqueue_step(bd_context *c, bool refactor) {
     // force pass 2 (3628-3646)
     enqueue_forces(c, c->pred, c->f1, nullptr, BD_TERM_ALL, 1);
     // corrector from x0 with the averaged forces, same rd and rr (3648-3652)
-    M(K_AVG, 0); launch_average(s, c->f0, c->f1, c->fa, q); M(K_AVG, 1);
-    M(K_ROTATE, 0); launch_rotate(s, c->cur, c->pred, c->fa.T, c->rr, q); M(K_ROTATE, 1);
+    M(K_AVG, 0); launch_average_rotate(s, c->f0, c->f1, c->fa, c->cur, c->pred, c->rr, q); M(K_AVG, 1);
     M(K_TRANSLATE, 0); launch_translate(s, c->hi_all, c->cur, c->pred, c->fa, c->rd, c->rad_all, q); M(K_TRANSLATE, 1);
     // commit and rebuild (3654-3660)
     M(K_COMMIT, 0); launch_commit(s, c->pred, c->cur, q); M(K_COMMIT, 1);

@@ -184,17 +184,16 @@ __global__ void k_betas(Stat s, Geo g, int do_t, int do_l) {
 // rotation_cal + rotate (func_cuda.cu:681-704, func.cpp:1607-1670): one
 // thread per bead, the increment of every bead formed before any frame is
 // written (the reference GPU kernel races here, func_cuda.cu:702).
-__global__ void k_rotate(Stat s, Geo src, Geo dst, const double *T, const double *rr) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= s.n) return;
+__device__ __forceinline__ void rotate_bead(const Stat &s, const Geo &src, const Geo &dst, int j, const double (&T)[3],
+                                            const double *rr) {
     const double zc = 8 * kPI * kETA * 125.0, zl = 4 * kPI * kETA * kRH * kRH * kLO;
     double wa, wb, wc;
     if (s.type[j] != 0) {
-        wa = s.time_step * T[3 * j] / zc + rr[3 * j];
-        wb = s.time_step * T[3 * j + 1] / zc + rr[3 * j + 1];
-        wc = s.time_step * T[3 * j + 2] / zc + rr[3 * j + 2];
+        wa = s.time_step * T[0] / zc + rr[3 * j];
+        wb = s.time_step * T[1] / zc + rr[3 * j + 1];
+        wc = s.time_step * T[2] / zc + rr[3 * j + 2];
     } else {
-        wa = s.time_step * T[3 * j] / zl + rr[3 * j];
+        wa = s.time_step * T[0] / zl + rr[3 * j];
         wb = 0.0;
         wc = 0.0;
     }
@@ -202,7 +201,9 @@ __global__ void k_rotate(Stat s, Geo src, Geo dst, const double *T, const double
     double *an = dst.a + 3 * j, *bn = dst.b + 3 * j, *cn = dst.c + 3 * j;
     double wa2 = wa * wa, wb2 = wb * wb, wc2 = wc * wc;
     double z2 = wa2 + wb2 + wc2, z = sqrt(z2);
-    double czt = cos(z), szt = sin(z), om = 1.0 - czt;
+    double czt, szt;
+    sincos_small(z, szt, czt);
+    const double om = 1.0 - czt;
     if (!(z2 > 0.0)) {
         for (int k = 0; k < 3; k++) { an[k] = a[k]; bn[k] = b[k]; cn[k] = c[k]; }
         return;
@@ -220,22 +221,30 @@ __global__ void k_rotate(Stat s, Geo src, Geo dst, const double *T, const double
         cn[k] = gc0 * A[k] + gc1 * B[k] + gc2 * C[k];
     }
 }
+__global__ void k_rotate(Stat s, Geo src, Geo dst, const double *T, const double *rr) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= s.n) return;
+    const double Tj[3] = {T[3 * j], T[3 * j + 1], T[3 * j + 2]};
+    rotate_bead(s, src, dst, j, Tj, rr);
+}
 
 // RPY 3x3 block between beads at ri and rj (func.cpp:752-799); rows are
 // the components of bead i.  Identical value for (j, i): only products of
-// pairs of components and squares of r_i - r_j enter.
+// pairs of components and squares of r_i - r_j enter.  The far-field
+// quotients by s and s^2 as products with one reciprocal of s.
 __device__ __forceinline__ void rpy_block(double3 ri, double3 rj, double radi, double radj, double a1,
                                           double a2, double M[3][3]) {
     double sij[3] = {ri.x - rj.x, ri.y - rj.y, ri.z - rj.z};
     double ssq = sij[0] * sij[0] + sij[1] * sij[1] + sij[2] * sij[2];
     double s = sqrt(ssq);
     if (s >= radi + radj) {
-        double f = (radi * radi + radj * radj) / ssq;
-        double f1 = 1.0 + f / 3, f2 = 1.0 - f;
-        double pre = a2 / s;
+        const double is = 1.0 / s, iss = is * is;
+        const double f = (radi * radi + radj * radj) * iss;
+        const double f1 = 1.0 + f / 3, f2i = (1.0 - f) * iss;
+        const double pre = a2 * is;
         for (int k = 0; k < 3; k++)
             for (int l = 0; l < 3; l++)
-                M[k][l] = (k == l) ? pre * (f1 + f2 * sij[k] * sij[k] / ssq) : pre * (f2 * sij[k] * sij[l] / ssq);
+                M[k][l] = (k == l) ? pre * (f1 + f2i * sij[k] * sij[k]) : pre * (f2i * sij[k] * sij[l]);
     } else {
         double se = pow((radi * radi * radi + radj * radj * radj) / 2.0, 1.0 / 3);
         double pre = a1 / se, sq = sqrt(ssq);
@@ -250,7 +259,9 @@ __device__ __forceinline__ void rpy_block(double3 ri, double3 rj, double radi, d
 // del * D_blk * F of a bead is summed over the beads of its species block
 // (main / tail / LH) with the RPY blocks recomputed in closed form -- the
 // same entries a row of the stored D holds.  One workgroup per bead of
-// the hydrodynamic set, threads over partners, fixed-order reduction.
+// the hydrodynamic set, threads over partners, fixed-order reduction; the
+// workgroups past n_hi take kTrThreads beads outside the set each
+// (self-mobility a1/rad only).
 constexpr int kTrThreads = 128;
 
 __device__ __forceinline__ void species(const Stat &s, const Geo &src, const Geo &dst, const Frc &f, int b,
@@ -260,13 +271,22 @@ __device__ __forceinline__ void species(const Stat &s, const Geo &src, const Geo
     else { x0 = src.r_lh; F = f.lF; out = dst.r_lh; lo = s.n + s.n_tail; cnt = s.n_LH; }
 }
 
-__global__ void __launch_bounds__(kTrThreads) k_translate_hi(Stat s, Geo src, Geo dst, Frc f, const double *rd,
-                                                            const double *rad_all) {
+__global__ void __launch_bounds__(kTrThreads) k_translate(Stat s, int n_hi, Geo src, Geo dst, Frc f,
+                                                         const double *rd, const double *rad_all) {
     __shared__ double red[3 * (kTrThreads / 64)];
     const int b = blockIdx.x;
     const double *x0, *F;
     double *out;
     int lo, cnt;
+    if (b >= n_hi) {
+        const int bd = n_hi + (b - n_hi) * kTrThreads + threadIdx.x;
+        if (bd >= s.n + s.n_tail + s.n_LH) return;
+        species(s, src, dst, f, bd, x0, F, out, lo, cnt);
+        const int i = bd - lo;
+        const double dself = s.a1 / rad_all[bd];
+        for (int k = 0; k < 3; k++) out[3 * i + k] = x0[3 * i + k] + rd[3 * bd + k] + s.del * dself * F[3 * i + k];
+        return;
+    }
     species(s, src, dst, f, b, x0, F, out, lo, cnt);
     const int i = b - lo;
     const double radi = rad_all[b];
@@ -289,26 +309,21 @@ __global__ void __launch_bounds__(kTrThreads) k_translate_hi(Stat s, Geo src, Ge
         for (int k = 0; k < 3; k++) out[3 * i + k] = (x0[3 * i + k] + rd[3 * b + k]) + v[k];
 }
 
-// beads outside the hydrodynamic set: self-mobility a1/rad only
-__global__ void k_translate_diag(Stat s, int first, Geo src, Geo dst, Frc f, const double *rd,
-                                 const double *rad_all) {
-    const int b = first + blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= s.n + s.n_tail + s.n_LH) return;
-    const double *x0, *F;
-    double *out;
-    int lo, cnt;
-    species(s, src, dst, f, b, x0, F, out, lo, cnt);
-    const int i = b - lo;
-    const double dself = s.a1 / rad_all[b];
-    for (int k = 0; k < 3; k++) out[3 * i + k] = x0[3 * i + k] + rd[3 * b + k] + s.del * dself * F[3 * i + k];
-}
-
-// force_torque_tmp (func_cuda.cu:2626-2641)
-__global__ void k_average(int n3, int nt3, int nl3, Frc a, Frc b, Frc o) {
+// force_torque_tmp (func_cuda.cu:2626-2641) and the corrector rotation
+// from x0 with the averaged torque: thread i averages element i of every
+// force array and rotates bead i (its torque averaged in place, the same
+// operations as the stored average).
+__global__ void k_average_rotate(Stat s, int n3, int nt3, int nl3, Frc a, Frc b, Frc o, Geo src, Geo dst,
+                                 const double *rr) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n3) { o.F[i] = 0.5 * (a.F[i] + b.F[i]); o.T[i] = 0.5 * (a.T[i] + b.T[i]); }
     if (i < nt3) o.tF[i] = 0.5 * (a.tF[i] + b.tF[i]);
     if (i < nl3) o.lF[i] = 0.5 * (a.lF[i] + b.lF[i]);
+    if (i < s.n) {
+        const double Tj[3] = {0.5 * (a.T[3 * i] + b.T[3 * i]), 0.5 * (a.T[3 * i + 1] + b.T[3 * i + 1]),
+                              0.5 * (a.T[3 * i + 2] + b.T[3 * i + 2])};
+        rotate_bead(s, src, dst, i, Tj, rr);
+    }
 }
 
 // final_updates (func_cuda.cu:2643-2662)
@@ -339,15 +354,15 @@ void launch_translate(const Stat &s, int hi_all, const Geo &src, const Geo &dst,
                       const double *rd, const double *rad_all, hipStream_t q) {
     const int tot = s.n + s.n_tail + s.n_LH;
     const int n_hi = hi_all ? tot : s.n;
-    k_translate_hi<<<n_hi, kTrThreads, 0, q>>>(s, src, dst, f, rd, rad_all);
-    if (tot > n_hi) k_translate_diag<<<nblk(tot - n_hi, 128), 128, 0, q>>>(s, n_hi, src, dst, f, rd, rad_all);
+    k_translate<<<n_hi + nblk(tot - n_hi, kTrThreads), kTrThreads, 0, q>>>(s, n_hi, src, dst, f, rd, rad_all);
 }
 
-void launch_average(const Stat &s, const Frc &a, const Frc &b, const Frc &out, hipStream_t q) {
+void launch_average_rotate(const Stat &s, const Frc &a, const Frc &b, const Frc &out, const Geo &src, const Geo &dst,
+                           const double *rr, hipStream_t q) {
     int mx = 3 * s.n;
     if (3 * s.n_tail > mx) mx = 3 * s.n_tail;
     if (3 * s.n_LH > mx) mx = 3 * s.n_LH;
-    k_average<<<nblk(mx, 256), 256, 0, q>>>(3 * s.n, 3 * s.n_tail, 3 * s.n_LH, a, b, out);
+    k_average_rotate<<<nblk(mx, 256), 256, 0, q>>>(s, 3 * s.n, 3 * s.n_tail, 3 * s.n_LH, a, b, out, src, dst, rr);
 }
 
 void launch_commit(const Stat &s, const Geo &from, const Geo &to, hipStream_t q) {

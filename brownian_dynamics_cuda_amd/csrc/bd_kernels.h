@@ -51,7 +51,10 @@ void launch_rotate(const Stat &s, const Geo &src, const Geo &dst, const double *
                    hipStream_t q);
 void launch_translate(const Stat &s, int hi_all, const Geo &src, const Geo &dst, const Frc &f,
                       const double *rd, const double *rad_all, hipStream_t q);
-void launch_average(const Stat &s, const Frc &a, const Frc &b, const Frc &out, hipStream_t q);
+// out = (a + b) / 2, and the rotation of every bead of src into dst with
+// the averaged torque
+void launch_average_rotate(const Stat &s, const Frc &a, const Frc &b, const Frc &out, const Geo &src, const Geo &dst,
+                           const double *rr, hipStream_t q);
 void launch_commit(const Stat &s, const Geo &from, const Geo &to, hipStream_t q);
 
 // diffusion tensor, Cholesky, noise (bd_chol.hip)
