@@ -18,8 +18,9 @@ namespace bd {
 // ---------------------------------------------------------------------------
 // world positions of core charges and globular-head sites
 // ---------------------------------------------------------------------------
-__global__ void k_sites(Stat s, Geo g, Sites st) {
+__global__ void k_sites(Stat s, Geo g, Sites st, int *cc_count) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *cc_count = 0;  // the core-core pair list of this pass starts empty
     int nq = s.n_c * s.Nq;
     if (t < nq) {
         int k = t / s.Nq, q = t % s.Nq, cb = s.core_bead[k];
@@ -880,39 +881,46 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
 
 // The core-core charge sums of core kc: its row (as c1) and column (as c2)
 // of the slot matrix that k_cc_work fills (func.cpp:1489-1566), one
-// workgroup per core, threads over the (other core, unit) slots in index
-// order and a fixed-order block sum; adds to F, T of the core bead and owns
+// workgroup per core: the partner cores of kc's pairs are listed in index
+// order, then threads take their (partner, unit) slots in that order and a
+// fixed-order block sum adds them; adds to F, T of the core bead and owns
 // the energy of the pairs where kc is c1.
 constexpr int kGatherThreads = 256;
 __global__ void __launch_bounds__(kGatherThreads) k_cc_gather(Stat s, Geo g, Frc f, const unsigned char *cc_valid,
                                                               int nchunk, const double *cc_slots, double *Ecc,
                                                               int terms) {
     __shared__ double red[10 * (kGatherThreads / 64)];
-    const int kc = blockIdx.x, lane = threadIdx.x;
+    __shared__ int s_list[kGatherThreads], s_wsum[kGatherThreads / 64];
+    const int kc = blockIdx.x, tid = threadIdx.x;
     const int cb = s.core_bead[kc];
     double3 F = zero3(), T = zero3(), Tw = zero3();
     double Ec = 0.0;
     // without the electrostatic terms only the energy slot is cleared
-    const int nx = (terms & TERM_ELEC) ? s.n_c * nchunk : 0;
-    for (int x = lane; x < nx; x += kGatherThreads) {
-        const int o = x / nchunk, ch = x % nchunk;
-        if (o == kc) continue;
-        const bool first = kc < o;
-        const size_t pidx = first ? (size_t)kc * s.n_c + o : (size_t)o * s.n_c + kc;
-        if (!cc_valid[pidx]) continue;
-        const double *sl = cc_slots + 10 * (pidx * nchunk + ch);
-        if (first) {
-            F += make_double3(sl[0], sl[1], sl[2]);
-            T += make_double3(sl[3], sl[4], sl[5]);
-            Ec += sl[9];
-        } else {
-            F -= make_double3(sl[0], sl[1], sl[2]);
-            Tw += make_double3(sl[6], sl[7], sl[8]);
+    const int no = (terms & TERM_ELEC) ? s.n_c : 0;
+    for (int base = 0; base < no; base += kGatherThreads) {
+        const int o = base + tid;
+        bool keep = false;
+        if (o < no && o != kc) keep = cc_valid[kc < o ? (size_t)kc * s.n_c + o : (size_t)o * s.n_c + kc] != 0;
+        const int cnt = compact_ordered<kGatherThreads>(keep, o, s_list, s_wsum);
+        for (int x = tid; x < cnt * nchunk; x += kGatherThreads) {
+            const int o2 = s_list[x / nchunk], ch = x % nchunk;
+            const bool first = kc < o2;
+            const size_t pidx = first ? (size_t)kc * s.n_c + o2 : (size_t)o2 * s.n_c + kc;
+            const double *sl = cc_slots + 10 * (pidx * nchunk + ch);
+            if (first) {
+                F += make_double3(sl[0], sl[1], sl[2]);
+                T += make_double3(sl[3], sl[4], sl[5]);
+                Ec += sl[9];
+            } else {
+                F -= make_double3(sl[0], sl[1], sl[2]);
+                Tw += make_double3(sl[6], sl[7], sl[8]);
+            }
         }
+        __syncthreads();
     }
     double v[10] = {F.x, F.y, F.z, T.x, T.y, T.z, Tw.x, Tw.y, Tw.z, Ec};
     block_sum_n<kGatherThreads, 10>(v, red);
-    if (lane == 0) {
+    if (tid == 0) {
         const double3 ac = ld3(g.a + 3 * cb), bc = ld3(g.b + 3 * cb), cc = ld3(g.c + 3 * cb);
         const double3 tw = make_double3(v[6], v[7], v[8]);
         double *Fc = f.F + 3 * cb, *Tc = f.T + 3 * cb;
@@ -1054,9 +1062,9 @@ __global__ void k_fill_i(int *p, long long n, int v) {
 // ---------------------------------------------------------------------------
 static inline int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
 
-void launch_sites(const Stat &s, const Geo &g, const Sites &st, hipStream_t q) {
+void launch_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, hipStream_t q) {
     long long tot = (long long)s.n_c * s.Nq + (long long)s.n_lhc * s.n_lh_g;
-    if (tot > 0) k_sites<<<nblk(tot, 256), 256, 0, q>>>(s, g, st);
+    k_sites<<<nblk(std::max(tot, 1LL), 256), 256, 0, q>>>(s, g, st, cc_count);
     const int nch = nchunks(s.n_tail) + nchunks(s.n_LH) + nchunks(s.n_lhc * s.n_lh_g) + nchunks(s.n_lnk);
     if (nch > 0) k_chunks<<<nblk(nch, 4), 256, 0, q>>>(s, g, st);
 }
@@ -1067,7 +1075,7 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
               off_core = off_lh + s.n_LH, off_cc = off_core + s.n_c;
     auto E = [&](int off) { return Epart ? Epart + off : nullptr; };
     auto M = [&](int id, int ph) { if (mark) mark(arg, id, ph); };
-    M(K_SITES, 0); launch_sites(s, g, st, q); M(K_SITES, 1);
+    M(K_SITES, 0); launch_sites(s, g, st, w.cc_count, q); M(K_SITES, 1);
     // k_mech assigns F, T of every main bead, then k_linker adds to the
     // linker beads and k_core to the cores (disjoint beads, so the two run
     // side by side); k_tail and k_lh assign their own arrays; k_cc_gather
@@ -1085,7 +1093,6 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
     if (terms & TERM_ELEC) {
         long long nn = (long long)s.n_c * s.n_c;
         M(K_CCLIST, 0);
-        k_fill_i<<<1, 64, 0, qc>>>(w.cc_count, 1, 0);
         k_cc_list<<<nblk(nn, 256), 256, 0, qc>>>(s, g, w.cc_valid, w.cc_list, w.cc_count);
         M(K_CCLIST, 1);
         M(K_CCWORK, 0);

@@ -21,7 +21,9 @@ struct ForceScratch {
 };
 
 // forces (bd_forces.hip)
-void launch_sites(const Stat &s, const Geo &g, const Sites &st, hipStream_t q);
+// world charge and head sites, partner chunk spheres; resets the core-core
+// pair count of the pass
+void launch_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, hipStream_t q);
 // mark(arg, id, phase): optional timing hook called around each kernel
 // (phase 0 before, 1 after); ids are the K_* values below.
 enum KernelId { K_SITES, K_MECH, K_LINKER, K_TAIL, K_LH, K_CORE, K_CCLIST, K_CCWORK,
