@@ -616,8 +616,14 @@ __global__ void __launch_bounds__(256) k_potrf0(double *A, int ld, double *Linv,
 template <int MODE>
 __global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, double *Linv, int *neg) {
     __shared__ double W[TB * LDP], As[TB * LDP], Bs[TB * LDP], Cs[TB * LDP];
+    // workgroups go round-robin over the eight XCDs (b % 8); XCD x takes a
+    // contiguous run of tiles, so the tiles of a block row, which read the
+    // same panel tile, share an L2 (workgroup 0, the diagonal tile, stays 0)
+    int lb = blockIdx.x;
+    const int G8 = gridDim.x / 8;
+    if (lb < 8 * G8) lb = (lb % 8) * G8 + lb / 8;
     int ii, jj;
-    tri_index(blockIdx.x, ii, jj);
+    tri_index(lb, ii, jj);
     const int ib = kb + 1 + ii, jb = kb + 1 + jj;
     const bool diag = ib == jb;
     {
