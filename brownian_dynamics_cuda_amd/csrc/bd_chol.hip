@@ -77,9 +77,18 @@ __device__ __forceinline__ void tri_index(int idx, int &i, int &j) {
     j = idx - r * (r + 1) / 2;
 }
 
+__device__ __forceinline__ void diag_factor_inv(double *T, double *LP, double *X, double *A, int ld, int d0,
+                                                double *Linv, int *neg);
+
+// FACTOR: workgroup 0 (tile (0,0)) then factors and inverts its tile in
+// place of a separate first-tile launch, beside the other tiles' assembly
+// (every workgroup then carries the factor's LDS).
+template <bool FACTOR>
 __global__ void __launch_bounds__(256) k_rpy(Stat s, Geo g, const double *rad_all, double *A, int m, int ld,
-                                             double a1, double a2) {
+                                             double a1, double a2, int *neg, double *Linv) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *neg = 0;  // negative-pivot count of the factor that follows
     __shared__ double T[TB * LDP];
+    __shared__ double LP[FACTOR ? TB * TB : 1], X[FACTOR ? TB * LDP : 1];
     int ti, tj;
     tri_index(blockIdx.x, ti, tj);
     const int r0 = ti * TB, c0 = tj * TB, nhi = m / 3;
@@ -111,13 +120,21 @@ __global__ void __launch_bounds__(256) k_rpy(Stat s, Geo g, const double *rad_al
         }
     }
     __syncthreads();
+    const bool fac = FACTOR && blockIdx.x == 0;
     for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
         const int row = r0 + e / TB, col = c0 + e % TB;
         double v;
         if (col > row) v = 0.0;
         else if (row >= m || col >= m) v = (row == col) ? 1.0 : 0.0;
         else v = T[(e / TB) * LDP + e % TB];
-        A[(size_t)row * ld + col] = v;
+        if (fac) T[(e / TB) * LDP + e % TB] = v;  // the factor writes L here instead
+        else A[(size_t)row * ld + col] = v;
+    }
+    if constexpr (FACTOR) {
+        if (fac) {
+            __syncthreads();
+            diag_factor_inv(T, LP, X, A, ld, 0, Linv, neg);
+        }
     }
 }
 
@@ -590,13 +607,6 @@ __device__ __forceinline__ void diag_factor_inv(double *T, double *LP, double *X
     store_xrows(48, 16, threadIdx.x, NTHR);
 }
 
-// first diagonal tile
-__global__ void __launch_bounds__(256) k_potrf0(double *A, int ld, double *Linv, int *neg) {
-    __shared__ double T[TB * LDP], LP[TB * TB], X[TB * LDP];
-    load_tile(A, ld, 0, 0, T);
-    __syncthreads();
-    diag_factor_inv(T, LP, X, A, ld, 0, Linv, neg);
-}
 
 // One block column kb of the factorisation in one launch, one workgroup per
 // trailing lower tile (ib, jb), kb < jb <= ib:
@@ -694,19 +704,19 @@ __global__ void __launch_bounds__(256) k_noise(const double *A, int ld, int m, i
 }
 
 // ---------------------------------------------------------------------------
-void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, const CholWork &c, hipStream_t q) {
+void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, const CholWork &c, hipStream_t q,
+                bool factor_first) {
     const int nt = c.mpad / TB;
-    k_rpy<<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2);
+    if (factor_first)
+        k_rpy<true><<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2, c.neg, c.Linv);
+    else
+        k_rpy<false><<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2, c.neg, c.Linv);
     (void)n_hi;
 }
 
 void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_arg, hipEvent_t gate, int gate_kb) {
     const int nt = c.mpad / TB;
     auto M_ = [&](int id, int ph) { if (mark) mark(mark_arg, id, ph); };
-    launch_fill_int(c.neg, 1, 0, q);
-    M_(K_POTRF, 0);
-    k_potrf0<<<1, 256, 0, q>>>(c.A, c.ld, c.Linv, c.neg);
-    M_(K_POTRF, 1);
     for (int kb = 0; kb + 1 < nt; kb++) {
         const int M = nt - kb - 1;
         M_(K_CHOL_STEP, 0);

@@ -55,7 +55,7 @@ int fail(int code, const char *fmt, ...) {
 
 const char *kKernelNames[K_COUNT] = {
     "sites", "mech", "linker", "tail", "lh", "core", "cc_list", "cc_work", "rpy",
-    "potrf", "chol_step", "noise", "frames", "betas", "rotate", "translate", "average",
+    "chol_step", "noise", "frames", "betas", "rotate", "translate", "average",
     "commit", "energy_reduce", "cc_gather"};
 
 }  // namespace
@@ -183,7 +183,7 @@ void enqueue_forces(bd_context *c, const Geo &g, const Frc &f, double *Epart, in
 void enqueue_factor(bd_context *c, const Geo &g, hipStream_t q, hipEvent_t gate = nullptr, int gate_kb = -1) {
     Marker M{c};
     M(K_RPY, 0);
-    launch_rpy(c->st, c->n_hi, g, c->rad_all, c->chol, q);
+    launch_rpy(c->st, c->n_hi, g, c->rad_all, c->chol, q, true);
     M(K_RPY, 1);
     launch_cholesky(c->chol, q, M.fn(), c, gate, gate_kb);
 }

@@ -27,7 +27,7 @@ void launch_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, h
 // mark(arg, id, phase): optional timing hook called around each kernel
 // (phase 0 before, 1 after); ids are the K_* values below.
 enum KernelId { K_SITES, K_MECH, K_LINKER, K_TAIL, K_LH, K_CORE, K_CCLIST, K_CCWORK,
-                K_RPY, K_POTRF, K_CHOL_STEP, K_NOISE, K_FRAMES, K_BETAS, K_ROTATE, K_TRANSLATE,
+                K_RPY, K_CHOL_STEP, K_NOISE, K_FRAMES, K_BETAS, K_ROTATE, K_TRANSLATE,
                 K_AVG, K_COMMIT, K_EREDUCE, K_CCGATHER, K_COUNT };
 typedef void (*MarkFn)(void *, int, int);
 // Extra streams of a force pass: after the charge sites, the tail, the
@@ -67,9 +67,12 @@ struct CholWork {
     int *neg;        // negative-pivot counter
     int mode;        // BD_CHOL_*
 };
+// D into c.A; also clears the negative-pivot count of the factor that
+// follows, and with factor_first factors and inverts the first diagonal tile
 void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, const CholWork &c,
-                hipStream_t q);
-// gate (optional) is recorded on q after the launch of block column gate_kb
+                hipStream_t q, bool factor_first = false);
+// the factor of the D that launch_rpy(factor_first) left in c.A; gate
+// (optional) is recorded on q after the launch of block column gate_kb
 // (after the last one when gate_kb is past the end)
 void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark = nullptr, void *mark_arg = nullptr,
                      hipEvent_t gate = nullptr, int gate_kb = -1);
