@@ -233,8 +233,12 @@ void enqueue_step(bd_context *c, bool refactor) {
     }
     M(K_TRANSLATE, 0); launch_translate(s, c->hi_all, c->cur, c->pred, c->f0, c->rd, c->rad_all, q); M(K_TRANSLATE, 1);
     // rebuild at the predicted state (3621-3625)
-    M(K_BETAS, 0); launch_betas(s, c->pred, 1, 1, q); M(K_BETAS, 1);
-    M(K_FRAMES, 0); launch_frames(s, c->pred, q); M(K_FRAMES, 1);
+    M(K_FRAMES, 0);
+    if (!launch_frames_betas(s, c->pred, 1, 1, q)) {
+        launch_betas(s, c->pred, 1, 1, q);
+        launch_frames(s, c->pred, q);
+    }
+    M(K_FRAMES, 1);
     // force pass 2 (3628-3646)
     enqueue_forces(c, c->pred, c->f1, nullptr, BD_TERM_ALL, 1);
     // corrector from x0 with the averaged forces, same rd and rr (3648-3652)
@@ -242,8 +246,12 @@ void enqueue_step(bd_context *c, bool refactor) {
     M(K_TRANSLATE, 0); launch_translate(s, c->hi_all, c->cur, c->pred, c->fa, c->rd, c->rad_all, q); M(K_TRANSLATE, 1);
     // commit and rebuild (3654-3660)
     M(K_COMMIT, 0); launch_commit(s, c->pred, c->cur, q); M(K_COMMIT, 1);
-    M(K_FRAMES, 0); launch_frames(s, c->cur, q); M(K_FRAMES, 1);
-    M(K_BETAS, 0); launch_betas(s, c->cur, 1, 1, q); M(K_BETAS, 1);
+    M(K_FRAMES, 0);
+    if (!launch_frames_betas(s, c->cur, 1, 1, q)) {
+        launch_frames(s, c->cur, q);
+        launch_betas(s, c->cur, 1, 1, q);
+    }
+    M(K_FRAMES, 1);
 }
 
 int alloc_geo(bd_context *c, Geo &g) {

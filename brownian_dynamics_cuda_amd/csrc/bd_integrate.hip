@@ -22,10 +22,8 @@ __device__ __forceinline__ double clamp1(double v) {
 // Only linker frames are written and only core frames are read across
 // beads, so one thread per bead has no race (the reference GPU kernel
 // fuses both passes and races, func_cuda.cu:722-957).
-__global__ void k_euler1(Stat s, Geo g) {
+__device__ __forceinline__ void euler1_bead(const Stat &s, const Geo &g, int i) {
 #pragma clang fp contract(off)
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.n - 1) return;
     const double si = sin(kTHETA), co = cos(kTHETA);
     double *ri = g.r + 3 * i, *rf = ri + 3, *ai = g.a + 3 * i, *bi = g.b + 3 * i, *ci = g.c + 3 * i;
     const double *af = ai + 3, *bf = bi + 3, *cf = ci + 3;
@@ -74,10 +72,8 @@ __device__ __forceinline__ double d6(const double *u, const double *v, const dou
 
 // update_Euler_Angle pass 2 (func.cpp:507-681): Euler angles and the exit
 // frame of every core.
-__global__ void k_euler2(Stat s, Geo g) {
+__device__ __forceinline__ void euler2_bead(const Stat &s, const Geo &g, int i) {
 #pragma clang fp contract(off)
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.n - 1) return;
     const double si = sin(kTHETA), co = cos(kTHETA);
     const double *ri = g.r + 3 * i, *rf = ri + 3, *ai = g.a + 3 * i, *bi = g.b + 3 * i, *ci = g.c + 3 * i;
     const double *af = ai + 3, *bf = bi + 3, *cf = ci + 3;
@@ -155,9 +151,8 @@ __global__ void k_euler2(Stat s, Geo g) {
 }
 
 // update_tail_beta (func.cpp:1763-1790) and update_LH_beta (2062-2089)
-__global__ void k_betas(Stat s, Geo g, int do_t, int do_l) {
+__device__ __forceinline__ void beta_site(const Stat &s, const Geo &g, int i, int do_t, int do_l) {
 #pragma clang fp contract(off)
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
     const double *p1, *p2, *p3;
     double *out;
     bool on;
@@ -179,6 +174,35 @@ __global__ void k_betas(Stat s, Geo g, int do_t, int do_l) {
     double x32 = p3[0] - p2[0], y32 = p3[1] - p2[1], z32 = p3[2] - p2[2];
     double r32 = sqrt(x32 * x32 + y32 * y32 + z32 * z32);
     *out = acos((x12 * x32 + y12 * y32 + z12 * z32) / (r12 * r32));
+}
+
+__global__ void k_euler1(Stat s, Geo g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < s.n - 1) euler1_bead(s, g, i);
+}
+__global__ void k_euler2(Stat s, Geo g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < s.n - 1) euler2_bead(s, g, i);
+}
+__global__ void k_betas(Stat s, Geo g, int do_t, int do_l) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < s.n_tail + s.n_LH) beta_site(s, g, i, do_t, do_l);
+}
+// Both frame passes and the tail/LH bending angles in one launch when the
+// chain has at most kFrameThreads + 1 beads: workgroup 0 runs pass 1 and,
+// after a barrier, pass 2 (pass 2 reads the linker frames of pass 1); the
+// other workgroups take the bending angles, which read positions only.
+constexpr int kFrameThreads = 1024;
+__global__ void __launch_bounds__(kFrameThreads) k_frames_betas(Stat s, Geo g, int do_t, int do_l) {
+    if (blockIdx.x == 0) {
+        const int i = threadIdx.x;
+        if (i < s.n - 1) euler1_bead(s, g, i);
+        __syncthreads();
+        if (i < s.n - 1) euler2_bead(s, g, i);
+        return;
+    }
+    const int i = (blockIdx.x - 1) * kFrameThreads + threadIdx.x;
+    if (i < s.n_tail + s.n_LH) beta_site(s, g, i, do_t, do_l);
 }
 
 // rotation_cal + rotate (func_cuda.cu:681-704, func.cpp:1607-1670): one
@@ -338,6 +362,13 @@ void launch_frames(const Stat &s, const Geo &g, hipStream_t q) {
     if (s.n < 2) return;
     k_euler1<<<nblk(s.n - 1, 64), 64, 0, q>>>(s, g);
     k_euler2<<<nblk(s.n - 1, 64), 64, 0, q>>>(s, g);
+}
+
+bool launch_frames_betas(const Stat &s, const Geo &g, int tails, int lh, hipStream_t q) {
+    if (s.n < 2 || s.n - 1 > kFrameThreads) return false;
+    const long long tot = (tails || lh) ? (long long)s.n_tail + s.n_LH : 0;
+    k_frames_betas<<<1 + nblk(tot, kFrameThreads), kFrameThreads, 0, q>>>(s, g, tails, lh);
+    return true;
 }
 
 void launch_betas(const Stat &s, const Geo &g, int tails, int lh, hipStream_t q) {

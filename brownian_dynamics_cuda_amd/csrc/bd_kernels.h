@@ -49,6 +49,9 @@ void launch_fill_int(int *p, long long n, int v, hipStream_t q);
 // frames and integrator (bd_integrate.hip)
 void launch_frames(const Stat &s, const Geo &g, hipStream_t q);         // update_Euler_Angle
 void launch_betas(const Stat &s, const Geo &g, int tails, int lh, hipStream_t q);
+// both, as one launch, for chains of up to 1025 beads; false (nothing
+// launched) for longer ones
+bool launch_frames_betas(const Stat &s, const Geo &g, int tails, int lh, hipStream_t q);
 void launch_rotate(const Stat &s, const Geo &src, const Geo &dst, const double *T, const double *rr,
                    hipStream_t q);
 void launch_translate(const Stat &s, int hi_all, const Geo &src, const Geo &dst, const Frc &f,
