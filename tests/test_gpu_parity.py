@@ -262,3 +262,49 @@ def test_mixed_precision_factor_displacement_error():
     err32 = np.linalg.norm(L32 @ L32.T - D) / np.linalg.norm(D)
     err64 = np.linalg.norm(L64 @ L64.T - D) / np.linalg.norm(D)
     assert err64 < 1e-14 and err32 < 1e-5
+
+
+def test_cfg3_forces_factor_and_long_chain():
+    """The 500-nucleosome fiber (D 10482 x 10482; a chain of 3494 beads,
+    longer than the one-workgroup frame launch takes): the frames and the
+    full force pass against the oracle, the factor through L (L^T v) = D v
+    on random vectors (an n^3 product on the host is out of reach), and two
+    steps replayed from a graph equal to two eager steps bit for bit."""
+    fib, st = Fb.make_config(3)
+    print("case: cfg3, 500 nucleosomes, D 10482 x 10482", flush=True)
+    orc = O.Oracle(fib)
+    fs = orc.prepare(st)
+    with E.BDEngine(fib, use_graph=False) as eng:
+        eng.set_state(st, rebuild=True)
+        g = eng.get_state()
+        d = eng.get_derived()
+        sync_engine(eng, fs)
+        got = eng.forces(E.TERM_ALL)
+        D = eng.get_tensor()
+        eng.factor()
+        L, neg = eng.get_factor()
+    for k in ("a", "b", "c"):
+        assert rel_err(getattr(g, k), fs.view(k)) <= 1e-14, k
+    for k in ("length", "a_dna", "b_dna", "c_dna", "beta_t"):
+        assert rel_err(d[k], fs.view(k)) <= 1e-13, k
+    ref = orc.forces(fs, ("all",))
+    for k in ("F", "T", "tF"):
+        assert rel_err(got[k], ref[k]) <= TOL, k
+    assert got["E"] == pytest.approx(ref["E"], rel=TOL)
+    assert neg == 0 and D.shape == (10482, 10482)
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        v = rng.standard_normal(D.shape[0])
+        Dv = D @ v
+        assert np.linalg.norm(L @ (L.T @ v) - Dv) / np.linalg.norm(Dv) <= 1e-13
+    del D, L
+    steps = [noise(fib, s) for s in range(2)]
+    out = {}
+    for graph in (True, False):
+        with E.BDEngine(fib, use_graph=graph) as eng:
+            eng.set_state(st, rebuild=True)
+            en = [eng.step(rr, p) for rr, p in steps]
+            out[graph] = (en, eng.get_state())
+    assert out[True][0] == out[False][0] and all(np.isfinite(out[True][0]))
+    for k in ("r", "a", "b", "c", "r_t"):
+        np.testing.assert_array_equal(getattr(out[True][1], k), getattr(out[False][1], k), err_msg=k)
