@@ -680,6 +680,76 @@ __global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, double 
         store_tile(A, ld, ib * TB, jb * TB, Cs, diag);
 }
 
+// The same block column in two launches, for wide trailing matrices (many
+// tiles per launch): k_panel forms every panel tile L_ib,kb once and parks
+// it in the upper block (kb, ib); k_update then updates each trailing tile
+// from two parked panel tiles, and its workgroup of tile (kb+1, kb+1)
+// factors and inverts it.  This trades one launch for the panel products
+// that k_step repeats in every workgroup of a block row (and for a smaller
+// LDS footprint: two panel workgroups fit on a CU).
+__global__ void __launch_bounds__(256) k_panel(double *A, int ld, int kb, const double *Linv) {
+    __shared__ double W[TB * LDP], As[TB * LDP];
+    const int ib = kb + 1 + blockIdx.x;
+    {
+        const double *const src[4] = {Linv + (size_t)kb * TB * TB, A + (size_t)ib * TB * ld + kb * TB, nullptr,
+                                      nullptr};
+        const int lds[4] = {TB, ld, 0, 0};
+        double *const dst[4] = {W, As, nullptr, nullptr};
+        load_tiles(2, src, lds, dst);
+    }
+    __syncthreads();
+    dbl4 pi[4];
+    for (int J = 0; J < 4; J++) pi[J] = dbl4{0.0, 0.0, 0.0, 0.0};
+    mma_panel(As, W, pi);
+    __syncthreads();
+    panel_to_lds(As, pi);
+    __syncthreads();
+    store_tile(A, ld, kb * TB, ib * TB, As, false);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_update(double *A, int ld, int kb, double *Linv, int *neg) {
+    __shared__ double As[TB * LDP], Bs[TB * LDP], Cs[TB * LDP];
+    int lb = blockIdx.x;  // XCD-contiguous runs of tiles, as in k_step
+    const int G8 = gridDim.x / 8;
+    if (lb < 8 * G8) lb = (lb % 8) * G8 + lb / 8;
+    int ii, jj;
+    tri_index(lb, ii, jj);
+    const int ib = kb + 1 + ii, jb = kb + 1 + jj;
+    const bool diag = ib == jb;
+    {
+        // the parked panel tiles L_ib,kb and L_jb,kb and the target tile
+        const double *const src[4] = {A + (size_t)kb * TB * ld + ib * TB, A + (size_t)ib * TB * ld + jb * TB,
+                                      A + (size_t)kb * TB * ld + jb * TB, nullptr};
+        const int lds[4] = {ld, ld, ld, 0};
+        double *const dst[4] = {As, Cs, Bs, nullptr};
+        load_tiles(diag ? 2 : 3, src, lds, dst);
+    }
+    __syncthreads();
+    if (MODE == 0) {
+        if (diag) {
+            syrk_lower(As, Cs);
+        } else {
+            dbl4 acc[2][2];
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+            mma_f64(As, Bs, acc);
+            acc_to_lds(Cs, acc, -1.0, false);
+        }
+    } else {
+        flt4 acc[2][2];
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) acc[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
+        mma_f32(As, diag ? As : Bs, acc);
+        acc_to_lds(Cs, acc, -1.0, false);
+    }
+    __syncthreads();
+    if (ii == 0 && jj == 0)
+        diag_factor_inv(Cs, As, Bs, A, ld, ib * TB, Linv + (size_t)ib * TB * TB, neg);
+    else
+        store_tile(A, ld, ib * TB, jb * TB, Cs, diag);
+}
+
 // rd = s2dt * L * p (rd_cal, func_cuda.cu:634-647): one wavefront per row,
 // lane k % 64 over the row; rows outside the hydrodynamic set use their
 // self-mobility: s2dt * sqrt(a1 / rad) * p.  L is read where the factor
@@ -714,16 +784,27 @@ void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, co
     (void)n_hi;
 }
 
+// block columns with at least this many trailing block rows take the
+// two-launch form (k_panel + k_update)
+constexpr int kSplitTiles = 40;
+
 void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_arg, hipEvent_t gate, int gate_kb) {
     const int nt = c.mpad / TB;
     auto M_ = [&](int id, int ph) { if (mark) mark(mark_arg, id, ph); };
     for (int kb = 0; kb + 1 < nt; kb++) {
         const int M = nt - kb - 1;
         M_(K_CHOL_STEP, 0);
-        if (c.mode == 0)
+        if (M >= kSplitTiles) {
+            k_panel<<<M, 256, 0, q>>>(c.A, c.ld, kb, c.Linv);
+            if (c.mode == 0)
+                k_update<0><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
+            else
+                k_update<1><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
+        } else if (c.mode == 0) {
             k_step<0><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
-        else
+        } else {
             k_step<1><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
+        }
         M_(K_CHOL_STEP, 1);
         if (gate && kb == gate_kb) (void)hipEventRecord(gate, q);
     }
