@@ -784,17 +784,13 @@ void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, co
     (void)n_hi;
 }
 
-// block columns with at least this many trailing block rows take the
-// two-launch form (k_panel + k_update)
-constexpr int kSplitTiles = 40;
-
 void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_arg, hipEvent_t gate, int gate_kb) {
     const int nt = c.mpad / TB;
     auto M_ = [&](int id, int ph) { if (mark) mark(mark_arg, id, ph); };
     for (int kb = 0; kb + 1 < nt; kb++) {
         const int M = nt - kb - 1;
         M_(K_CHOL_STEP, 0);
-        if (M >= kSplitTiles) {
+        if (M * (M + 1) / 2 >= c.split_tiles) {
             k_panel<<<M, 256, 0, q>>>(c.A, c.ld, kb, c.Linv);
             if (c.mode == 0)
                 k_update<0><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);

@@ -505,6 +505,13 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     c->chol.mpad = ((c->chol.m + 63) / 64) * 64;
     c->chol.ld = ((3 * nhi_cap + 63) / 64) * 64;
     c->chol.mode = o.chol_mode;
+    {
+        int ncu = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, o.device) == hipSuccess && prop.multiProcessorCount > 0)
+            ncu = prop.multiProcessorCount;
+        c->chol.split_tiles = ncu * 19 / 10;
+    }
     r |= c->alloc(&c->chol.A, (size_t)c->chol.ld * c->chol.ld);
     r |= c->alloc(&c->chol.Linv, (size_t)(c->chol.ld / 64) * 64 * 64);
     r |= c->alloc(&c->chol.neg, 1);

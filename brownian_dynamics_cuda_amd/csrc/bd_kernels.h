@@ -69,6 +69,10 @@ struct CholWork {
     double *Linv;    // (ld/64) tiles of 64x64: inverses of the diagonal factors, row-major
     int *neg;        // negative-pivot counter
     int mode;        // BD_CHOL_*
+    // block columns with at least this many trailing tiles take the
+    // two-launch form (k_panel + k_update): about two rounds of workgroups
+    // at one per CU
+    int split_tiles;
 };
 // D into c.A; also clears the negative-pivot count of the factor that
 // follows, and with factor_first factors and inverts the first diagonal tile
