@@ -1079,7 +1079,8 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
     // k_mech assigns F, T of every main bead, then k_linker adds to the
     // linker beads and k_core to the cores (disjoint beads, so the two run
     // side by side); k_tail and k_lh assign their own arrays; k_cc_gather
-    // adds k_cc_work's slots to the cores after the join
+    // adds k_cc_work's slots to the cores after the join.  Enqueue order:
+    // core-core, mechanics, linker, core, then tail and LH.
     hipStream_t qt = q, ql = q, qc = q;
     if (fs) {
         (void)hipEventRecord(fs->fork, q);
@@ -1088,8 +1089,6 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
         ql = fs->b[1];
         qc = fs->b[2];
     }
-    if (s.n_tail) { M(K_TAIL, 0); k_tail<<<s.n_tail, kTailThreads, 0, qt>>>(s, g, st, f, E(off_tail), terms); M(K_TAIL, 1); }
-    if (s.n_LH) { M(K_LH, 0); k_lh<<<s.n_LH, kLHThreads, 0, ql>>>(s, g, st, f, E(off_lh), terms); M(K_LH, 1); }
     if (terms & TERM_ELEC) {
         long long nn = (long long)s.n_c * s.n_c;
         M(K_CCLIST, 0);
@@ -1115,6 +1114,10 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
     k_core<<<dim3(s.n_c, cs[0] + cs[1] + cs[2]), kCoreThreads, 0, q>>>(s, g, st, f, w.core_part, w.core_cnt,
                                                                       E(off_core), terms, cs[0], cs[1]);
     M(K_CORE, 1);
+    // the tail and LH kernels go last: enqueued before the others, their many
+    // small workgroups took the CUs first (measured ~1% slower per step)
+    if (s.n_tail) { M(K_TAIL, 0); k_tail<<<s.n_tail, kTailThreads, 0, qt>>>(s, g, st, f, E(off_tail), terms); M(K_TAIL, 1); }
+    if (s.n_LH) { M(K_LH, 0); k_lh<<<s.n_LH, kLHThreads, 0, ql>>>(s, g, st, f, E(off_lh), terms); M(K_LH, 1); }
     if (fs) {
         for (int i = 0; i < 3; i++) {
             (void)hipEventRecord(fs->join[i], fs->b[i]);
