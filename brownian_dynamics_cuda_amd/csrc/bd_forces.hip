@@ -1075,50 +1075,64 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
               off_core = off_lh + s.n_LH, off_cc = off_core + s.n_c;
     auto E = [&](int off) { return Epart ? Epart + off : nullptr; };
     auto M = [&](int id, int ph) { if (mark) mark(arg, id, ph); };
-    M(K_SITES, 0); launch_sites(s, g, st, w.cc_count, q); M(K_SITES, 1);
     // k_mech assigns F, T of every main bead, then k_linker adds to the
     // linker beads and k_core to the cores (disjoint beads, so the two run
     // side by side); k_tail and k_lh assign their own arrays; k_cc_gather
-    // adds k_cc_work's slots to the cores after the join.  Enqueue order:
-    // core-core, mechanics, linker, core, then tail and LH.
-    hipStream_t qt = q, ql = q, qc = q;
-    if (fs) {
-        (void)hipEventRecord(fs->fork, q);
-        for (int i = 0; i < 3; i++) (void)hipStreamWaitEvent(fs->b[i], fs->fork, 0);
-        qt = fs->b[0];
-        ql = fs->b[1];
-        qc = fs->b[2];
-    }
-    if (terms & TERM_ELEC) {
-        long long nn = (long long)s.n_c * s.n_c;
+    // adds k_cc_work's slots to the cores last
+    auto tail = [&](hipStream_t qq) {
+        if (s.n_tail) { M(K_TAIL, 0); k_tail<<<s.n_tail, kTailThreads, 0, qq>>>(s, g, st, f, E(off_tail), terms); M(K_TAIL, 1); }
+    };
+    auto lh = [&](hipStream_t qq) {
+        if (s.n_LH) { M(K_LH, 0); k_lh<<<s.n_LH, kLHThreads, 0, qq>>>(s, g, st, f, E(off_lh), terms); M(K_LH, 1); }
+    };
+    auto cc = [&](hipStream_t qq) {
+        if (!(terms & TERM_ELEC)) return;
         M(K_CCLIST, 0);
-        k_cc_list<<<nblk(nn, 256), 256, 0, qc>>>(s, g, w.cc_valid, w.cc_list, w.cc_count);
+        k_cc_list<<<nblk((long long)s.n_c * s.n_c, 256), 256, 0, qq>>>(s, g, w.cc_valid, w.cc_list, w.cc_count);
         M(K_CCLIST, 1);
         M(K_CCWORK, 0);
         if (Epart)
-            k_cc_work<true><<<w.cc_grid, kCCThreads, 0, qc>>>(s, g, st.Qp, w.cc_list, w.cc_count, w.cc_nkb,
+            k_cc_work<true><<<w.cc_grid, kCCThreads, 0, qq>>>(s, g, st.Qp, w.cc_list, w.cc_count, w.cc_nkb,
                                                               w.cc_nlc, w.cc_slots, kExpCoef);
         else
-            k_cc_work<false><<<w.cc_grid, kCCThreads, 0, qc>>>(s, g, st.Qp, w.cc_list, w.cc_count, w.cc_nkb,
+            k_cc_work<false><<<w.cc_grid, kCCThreads, 0, qq>>>(s, g, st.Qp, w.cc_list, w.cc_count, w.cc_nkb,
                                                                w.cc_nlc, w.cc_slots, kExpCoef);
         M(K_CCWORK, 1);
-    }
-    M(K_MECH, 0); k_mech<<<nblk(s.n, 64), 64, 0, q>>>(s, g, f, E(off_mech), terms); M(K_MECH, 1);
-    if (fs) {
+    };
+    auto mech = [&]() { M(K_MECH, 0); k_mech<<<nblk(s.n, 64), 64, 0, q>>>(s, g, f, E(off_mech), terms); M(K_MECH, 1); };
+    auto linker = [&](hipStream_t qq) {
+        if (s.n_lnk) { M(K_LINKER, 0); k_linker<<<s.n_lnk, kLinkerThreads, 0, qq>>>(s, g, st, f, E(off_lnk), terms); M(K_LINKER, 1); }
+    };
+    auto core = [&]() {
+        M(K_CORE, 0);
+        const int *cs = w.core_split;
+        k_core<<<dim3(s.n_c, cs[0] + cs[1] + cs[2]), kCoreThreads, 0, q>>>(s, g, st, f, w.core_part, w.core_cnt,
+                                                                          E(off_core), terms, cs[0], cs[1]);
+        M(K_CORE, 1);
+    };
+    M(K_SITES, 0); launch_sites(s, g, st, w.cc_count, q); M(K_SITES, 1);
+    // The enqueue orders were measured (cfg2): on one stream, beside the
+    // factorisation, tail and LH first and core-core last; with branches,
+    // core-core first and tail and LH last (enqueued first there, their
+    // thousands of small workgroups took the CUs ahead of the longer chains).
+    if (!fs) {
+        tail(q);
+        lh(q);
+        mech();
+        linker(q);
+        core();
+        cc(q);
+    } else {
+        (void)hipEventRecord(fs->fork, q);
+        for (int i = 0; i < 3; i++) (void)hipStreamWaitEvent(fs->b[i], fs->fork, 0);
+        cc(fs->b[2]);
+        mech();
         (void)hipEventRecord(fs->mech, q);
-        (void)hipStreamWaitEvent(ql, fs->mech, 0);
-    }
-    if (s.n_lnk) { M(K_LINKER, 0); k_linker<<<s.n_lnk, kLinkerThreads, 0, ql>>>(s, g, st, f, E(off_lnk), terms); M(K_LINKER, 1); }
-    M(K_CORE, 0);
-    const int *cs = w.core_split;
-    k_core<<<dim3(s.n_c, cs[0] + cs[1] + cs[2]), kCoreThreads, 0, q>>>(s, g, st, f, w.core_part, w.core_cnt,
-                                                                      E(off_core), terms, cs[0], cs[1]);
-    M(K_CORE, 1);
-    // the tail and LH kernels go last: enqueued before the others, their many
-    // small workgroups took the CUs first (measured ~1% slower per step)
-    if (s.n_tail) { M(K_TAIL, 0); k_tail<<<s.n_tail, kTailThreads, 0, qt>>>(s, g, st, f, E(off_tail), terms); M(K_TAIL, 1); }
-    if (s.n_LH) { M(K_LH, 0); k_lh<<<s.n_LH, kLHThreads, 0, ql>>>(s, g, st, f, E(off_lh), terms); M(K_LH, 1); }
-    if (fs) {
+        (void)hipStreamWaitEvent(fs->b[1], fs->mech, 0);
+        linker(fs->b[1]);
+        core();
+        tail(fs->b[0]);
+        lh(fs->b[1]);
         for (int i = 0; i < 3; i++) {
             (void)hipEventRecord(fs->join[i], fs->b[i]);
             (void)hipStreamWaitEvent(q, fs->join[i], 0);
