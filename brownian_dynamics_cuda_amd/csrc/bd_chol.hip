@@ -346,43 +346,35 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 // ---------------------------------------------------------------------------
 
 // g[C0 .. C0+NC) -= sum_q x[q] * lp[col*8 + q] for the columns col =
-// colbase + C0 + c.  Two columns at a time, their FMA chains interleaved;
-// the entries of the next pair are loaded one pair ahead (a scheduling
-// barrier per pair keeps that distance).
+// colbase + C0 + c, each column's sum in increasing q.  Four columns at a
+// time, their FMA chains interleaved (with two chains the dependent FP64
+// FMA latency, not the issue rate, set the pace); the entries of the next
+// four columns (broadcast 16-byte LDS loads) are loaded one group ahead (a
+// scheduling barrier per group keeps that distance).
 template <int C0, int NC>
 __device__ __forceinline__ void panel_update16(double (&g)[16], const double (&x)[8], const double *lp,
                                                int colbase) {
-    const double2 *l2 = reinterpret_cast<const double2 *>(lp) + colbase * 4;
-    double2 p0[4], p1[4];
+    constexpr int NG = NC / 4;
+    const double2 *l2 = reinterpret_cast<const double2 *>(lp) + (colbase + C0) * 4;
+    double2 e[NG][16];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        p0[k] = l2[C0 * 4 + k];
-        p1[k] = l2[(C0 + 1) * 4 + k];
-    }
+    for (int k = 0; k < 16; k++) e[0][k] = l2[k];
 #pragma unroll
-    for (int c = C0; c < C0 + NC; c += 2) {
-        double2 u[4], v[4];
+    for (int i = 0; i < NG; i++) {
+        if (i + 1 < NG)
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            u[k] = p0[k];
-            v[k] = p1[k];
-        }
-        if (c + 3 < C0 + NC)
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                p0[k] = l2[(c + 2) * 4 + k];
-                p1[k] = l2[(c + 3) * 4 + k];
-            }
-        double s0 = g[c], s1 = g[c + 1];
+            for (int k = 0; k < 16; k++) e[i + 1][k] = l2[(i + 1) * 16 + k];
+        const int c = C0 + 4 * i;
+        double s[4] = {g[c], g[c + 1], g[c + 2], g[c + 3]};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            s0 -= x[2 * k] * u[k].x;
-            s1 -= x[2 * k] * v[k].x;
-            s0 -= x[2 * k + 1] * u[k].y;
-            s1 -= x[2 * k + 1] * v[k].y;
+#pragma unroll
+            for (int m = 0; m < 4; m++) s[m] -= x[2 * k] * e[i][4 * m + k].x;
+#pragma unroll
+            for (int m = 0; m < 4; m++) s[m] -= x[2 * k + 1] * e[i][4 * m + k].y;
         }
-        g[c] = s0;
-        g[c + 1] = s1;
+#pragma unroll
+        for (int m = 0; m < 4; m++) g[c + m] = s[m];
         __builtin_amdgcn_sched_barrier(0);
     }
 }
@@ -429,25 +421,67 @@ __device__ __forceinline__ void panel_row(const double *LP, double (&x)[8]) {
     }
 }
 
-// X_ww = inv(L_ww) for the 16x16 diagonal block w, by wavefront w: lane r
-// (r < 16) solves x L_ww^T = e_r, whose solution is column r of X_ww.
-__device__ __forceinline__ void diag_block_inverse(const double *LP, const double *ivs, double *X, int w) {
-    const int lane = threadIdx.x & 63, r = lane & 15;
-    const double *lw = LP + w * (2 * TB * 8 + 16 * 8);  // L[16w + c][16w + j] = lw[(j/8)*512 + c*8 + j%8]
-    const double *iw = ivs + 16 * w;
-    double b[16];
+// X_ww = inv(L_ww) for the 16x16 diagonal block w, by wavefront w, in 8x8
+// blocks: L_ww = [[A, 0], [B, C]] gives X_ww = [[A^-1, 0], [-C^-1 B A^-1, C^-1]].
+// Lanes 0-7 solve A y = e_r and lanes 8-15 C y = e_r by substitution, side
+// by side (y_c = (e_r[c] - sum_{j<c} L_cj y_j) / L_cc, eight steps); then
+// every lane forms one entry of Y = B A^-1 and one of -C^-1 Y.  The upper
+// triangle and the upper right block are written as zeros (the products
+// that read X_ww take whole 16x16 blocks).  S: the wavefront's LDS scratch
+// (8 rows of pitch 17).
+__device__ __forceinline__ void diag_block_inverse(const double *LP, const double *ivs, double *X, double *S, int w) {
+    const int lane = threadIdx.x & 63;
+    {
+        const int h = (lane >> 3) & 1, r = lane & 7;
+        const int b0 = 16 * w + 8 * h;  // first row and column of block A (h = 0) or C (h = 1)
+        // L[b0 + c][b0 + j] = lh[c*8 + j]: panel 2w + h, rows b0..b0+7
+        const double *lh = LP + (2 * w + h) * (TB * 8) + b0 * 8;
+        const double2 *i2 = reinterpret_cast<const double2 *>(ivs + b0);
+        double iw[8];
 #pragma unroll
-    for (int c = 0; c < 16; c++) b[c] = (c == r) ? 1.0 : 0.0;
+        for (int k = 0; k < 4; k++) {
+            const double2 v = i2[k];
+            iw[2 * k] = v.x;
+            iw[2 * k + 1] = v.y;
+        }
+        double y[8];
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-        const double xj = b[j] * iw[j];
-        b[j] = xj;
+        for (int c = 0; c < 8; c++) {
+            double l[8];
 #pragma unroll
-        for (int c = j + 1; c < 16; c++) b[c] -= xj * lw[(j >> 3) * (TB * 8) + c * 8 + (j & 7)];
+            for (int j = 0; j < c; j += 2) {
+                const double2 v = *reinterpret_cast<const double2 *>(lh + c * 8 + j);
+                l[j] = v.x;
+                l[j + 1] = v.y;
+            }
+            double sum = (c == r) ? 1.0 : 0.0;
+#pragma unroll
+            for (int j = 0; j < c; j++) sum -= y[j] * l[j];
+            y[c] = sum * iw[c];
+        }
+        if (lane < 16) {
+#pragma unroll
+            for (int c = 0; c < 8; c++) X[(b0 + c) * LDP + b0 + r] = y[c];
+            if (h)
+#pragma unroll
+                for (int c = 0; c < 8; c++) X[(16 * w + c) * LDP + b0 + r] = 0.0;
+        }
     }
-    if (lane < 16)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int a = lane >> 3, b = lane & 7, d0 = 16 * w;
+    // Y[a][b] = sum_k B[a][k] A^-1[k][b], B[a][k] = L[d0 + 8 + a][d0 + k] (panel 2w)
+    const double *lb = LP + 2 * w * (TB * 8) + (d0 + 8 + a) * 8;
+    double yv = 0.0;
 #pragma unroll
-        for (int c = 0; c < 16; c++) X[(16 * w + c) * LDP + 16 * w + r] = b[c];
+    for (int k = 0; k < 8; k++) yv += lb[k] * X[(d0 + k) * LDP + d0 + b];
+    S[a * 17 + b] = yv;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    double z = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) z += X[(d0 + 8 + a) * LDP + d0 + 8 + k] * S[k * 17 + b];
+    X[(d0 + 8 + a) * LDP + d0 + b] = -z;
 }
 
 // 16x16 block products on the matrix cores (v_mfma_f64_16x16x4f64; result
@@ -492,18 +526,45 @@ __device__ __forceinline__ void x_block(const double *LP, double *X, double *S, 
     const dbl4 z = {0.0, 0.0, 0.0, 0.0};
     put_block(X + 16 * i * LDP + 16 * j, LDP, mm_xs(z, X, S, i), -1.0);
 }
-// S = sum_{k=j}^{2} L_3k X_kj: the part of X_3j that does not need X_33
-__device__ __forceinline__ void s3_block(const double *LP, const double *X, double *S, int j) {
+// S = sum_{k=j}^{i-1} L_ik X_kj (the sum of X_ij = -X_ii S without X_ii)
+__device__ __forceinline__ void x_sum(const double *LP, const double *X, double *S, int i, int j) {
     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k = j; k < 3; k++) acc = mm_lx(acc, LP, X, 3, k, j);
+    for (int k = j; k < i; k++) acc = mm_lx(acc, LP, X, i, k, j);
     put_block(S, 17, acc, 1.0);
 }
+// X_ij = -X_ii S from a sum S formed by x_sum (possibly by another wavefront)
+__device__ __forceinline__ void x_final(double *X, const double *S, int i, int j) {
+    const dbl4 z = {0.0, 0.0, 0.0, 0.0};
+    put_block(X + 16 * i * LDP + 16 * j, LDP, mm_xs(z, X, S, i), -1.0);
+}
+// S (+)= sum_{k=k0}^{k1-1} L_3k X_kj: terms of the part of X_3j that does
+// not need X_33, S_3j = sum_{k=j}^{2} L_3k X_kj; with ADD the sum continues
+// from the partial sum already in S (the same order as in one pass).
+__device__ __forceinline__ void s3_part(const double *LP, const double *X, double *S, int j, int k0, int k1,
+                                        bool add) {
+    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    if (add)
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[v] = S[(k4 + 4 * v) * 17 + i16];
+    for (int k = k0; k < k1; k++) acc = mm_lx(acc, LP, X, 3, k, j);
+    put_block(S, 17, acc, 1.0);
+}
+// Rows r0 .. r0+15 of inv(L) from X to global memory (row-major 64x64), with
+// zeros in the 16x16 blocks above the diagonal; one wavefront.
+__device__ __forceinline__ void store_xrows(const double *X, double *Linv, int r0) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int e = lane; e < 16 * (TB / 2); e += 64) {
+        const int r = r0 + e / (TB / 2), c = 2 * (e % (TB / 2));
+        dbl2 v;
+        const bool keep = (r >> 4) >= (c >> 4);
+        v.x = keep ? X[r * LDP + c] : 0.0;
+        v.y = keep ? X[r * LDP + c + 1] : 0.0;
+        *reinterpret_cast<dbl2 *>(Linv + r * TB + c) = v;
+    }
+}
 
-// One panel step of the factor (all four wavefronts; ends with a barrier).
-// Wavefronts without columns left build inv(L) meanwhile: X_00, X_11, X_22
-// as soon as their columns are final, X_10 (step 5), X_21, X_20 and
-// S_32 = L_32 X_22 (step 7); S: the wavefront's 16x17 scratch, S3: the
-// three partial sums S_3j.
 // This lane's 16 columns of row d0 + lane of L to global memory (zeros above
 // the diagonal: a lane above a pivot column takes 0 at that pivot).
 __device__ __forceinline__ void store_lrow(const double (&g)[16], double *row) {
@@ -516,9 +577,17 @@ __device__ __forceinline__ void store_lrow(const double (&g)[16], double *row) {
     }
 }
 
+// One panel step of the factor (all four wavefronts; ends with a barrier).
+// Wavefronts without columns left build inv(L) meanwhile, each piece as
+// soon as its inputs are final: X_00 (step 2), X_11 (4), X_10 (5), X_22 (6),
+// the sums S_20, S_21 of X_20, X_21 and the k < 2 terms of S_30, S_31 (6),
+// X_21 = -X_22 S_21, X_20 = -X_22 S_20 and S_32 = L_32 X_22 (7);
+// rows 0-15 (4) and 16-31 (6) of inv(L) to global memory.  S: the
+// wavefront's 16x17 scratch, S3: the three partial sums S_3j.
 template <int P>
 __device__ __forceinline__ void factor_step(double (&g)[16], double *LP, double *ivs, double *X, double *S,
-                                            double *S3, double &myinv, int &nneg, int w, double *lrow) {
+                                            double *S3, double &myinv, int &nneg, int w, double *lrow,
+                                            double *Linv) {
     constexpr int O = P >> 1, LOC = P & 1;
     if (w == O) {
         if constexpr (LOC == 0 && P > 0) {
@@ -546,13 +615,25 @@ __device__ __forceinline__ void factor_step(double (&g)[16], double *LP, double 
             panel_update16<0, 16>(g, x, LP + (P - 1) * (TB * 8), 16 * w);
         }
     } else if (LOC == 0 && w == O - 1) {
-        diag_block_inverse(LP, ivs, X, w);
+        diag_block_inverse(LP, ivs, X, S, w);
+    } else if (P == 4 && w == 0) {
+        store_xrows(X, Linv, 0);
     } else if (P == 5 && w == 0) {
         x_block(LP, X, S, 1, 0);
+    } else if (P == 6) {
+        // S = T + 64 + w*272: wavefront 0's scratch takes S_20, wavefront 1's S_21
+        if (w == 0) {
+            store_xrows(X, Linv, 16);
+            x_sum(LP, X, S, 2, 0);
+        } else {
+            x_sum(LP, X, S, 2, 1);
+            s3_part(LP, X, S3, 0, 0, 2, false);
+            s3_part(LP, X, S3 + 16 * 17, 1, 1, 2, false);
+        }
     } else if (P == 7) {
-        if (w == 0) s3_block(LP, X, S3 + 2 * (16 * 17), 2);
-        else if (w == 1) x_block(LP, X, S, 2, 1);
-        else x_block(LP, X, S, 2, 0);
+        if (w == 0) s3_part(LP, X, S3 + 2 * (16 * 17), 2, 2, 3, false);
+        else if (w == 1) x_final(X, S, 2, 1);
+        else x_final(X, S - 2 * (16 * 17), 2, 0);
     }
     __syncthreads();
 }
@@ -572,39 +653,47 @@ __device__ __forceinline__ void diag_factor_inv(double *T, double *LP, double *X
     double *lrow = A + (size_t)(d0 + lane) * ld + d0 + 16 * w;
     int nneg = 0;
     double myinv = 0.0;
-    factor_step<0>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
-    factor_step<1>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
-    factor_step<2>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
-    factor_step<3>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
-    factor_step<4>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
-    factor_step<5>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
-    factor_step<6>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
-    factor_step<7>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow);
+    factor_step<0>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow, Linv);
+    factor_step<1>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow, Linv);
+    factor_step<2>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow, Linv);
+    factor_step<3>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow, Linv);
+    factor_step<4>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow, Linv);
+    factor_step<5>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow, Linv);
+    factor_step<6>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow, Linv);
+    factor_step<7>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow, Linv);
     if (nneg && lane == 0) atomicAdd(neg, nneg);
-    // rows 0..47 of inv(L) are final: wavefronts 0-2 store their stripe
-    // (zeros in the blocks above the diagonal) beside X_33 and S_30, S_31
-    auto store_xrows = [&](int r0, int nr, int tid, int nthr) {
-        for (int e = tid; e < nr * (TB / 2); e += nthr) {
-            const int r = r0 + e / (TB / 2), c = 2 * (e % (TB / 2));
+    // X_33, the k = 2 terms of S_30 and S_31, and rows 32-47 of inv(L)
+    if (w == 3) {
+        diag_block_inverse(LP, ivs, X, S, 3);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // X_33 to rows 48-63, columns 48-63 of inv(L)
+#pragma unroll
+        for (int e = lane; e < 16 * 8; e += 64) {
+            const int r = 48 + e / 8, c = 48 + 2 * (e % 8);
             dbl2 v;
-            const bool keep = (r >> 4) >= (c >> 4);
-            v.x = keep ? X[r * LDP + c] : 0.0;
-            v.y = keep ? X[r * LDP + c + 1] : 0.0;
+            v.x = X[r * LDP + c];
+            v.y = X[r * LDP + c + 1];
             *reinterpret_cast<dbl2 *>(Linv + r * TB + c) = v;
         }
-    };
-    if (w == 3) diag_block_inverse(LP, ivs, X, 3);
-    else if (w == 0) s3_block(LP, X, S3, 0);
-    else if (w == 1) s3_block(LP, X, S3 + 16 * 17, 1);
-    if (w < 3) store_xrows(16 * w, 16, lane, 64);
-    __syncthreads();
-    // X_3j = -X_33 S_3j, then the last stripe of inv(L)
-    if (w < 3) {
-        const dbl4 z = {0.0, 0.0, 0.0, 0.0};
-        put_block(X + 48 * LDP + 16 * w, LDP, mm_xs(z, X, S3 + w * (16 * 17), 3), -1.0);
+    } else if (w == 0) {
+        s3_part(LP, X, S3, 0, 2, 3, true);
+    } else if (w == 1) {
+        s3_part(LP, X, S3 + 16 * 17, 1, 2, 3, true);
+    } else {
+        store_xrows(X, Linv, 32);
     }
     __syncthreads();
-    store_xrows(48, 16, threadIdx.x, NTHR);
+    // X_3j = -X_33 S_3j straight from the matrix-core registers to rows
+    // 48-63, columns 16j..16j+15 of inv(L) (register v of lane l: element
+    // (l/16 + 4v, l%16))
+    if (w < 3) {
+        const dbl4 z = {0.0, 0.0, 0.0, 0.0};
+        const dbl4 x3 = mm_xs(z, X, S3 + w * (16 * 17), 3);
+        const int i16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+        for (int v = 0; v < 4; v++) Linv[(48 + k4 + 4 * v) * TB + 16 * w + i16] = -x3[v];
+    }
 }
 
 
