@@ -329,19 +329,19 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 // ---------------------------------------------------------------------------
 // Diagonal tile: factor and inverse by the four wavefronts of a workgroup.
 //
-// Wavefront w holds columns 16w..16w+15 of every row (lane = row) in g[].
-// The 8 panels of 8 columns are factored in order by the wavefront that
-// owns them (panels 2w and 2w+1): the CPU algorithm of func.cpp:806-836 in
-// right-looking order -- pivot f (|f| and a count if f < 0), column scaled
-// by 1/sqrt(f) -- with the updates inside a panel taken from the owning
-// lanes by readlane.  A finished panel goes to LP in panel order,
-// LP[p][row][q] = L[row][8p + q], and while its owner goes on, the
-// wavefronts to its right apply it to their columns (rank-8 updates that
-// read the 8 entries of a column as broadcast 16-byte LDS loads).  Only
-// the owner's pivots, its catch-up on the previous group's last panel and
-// its in-group update are on the critical path.  A wavefront with no
-// columns left inverts its 16x16 diagonal block of L by substitution, and
-// the off-diagonal 16x16 blocks of X = inv(L) follow on the matrix cores,
+// Wavefront w holds 16 columns of every row (lane = row) in g[]: panels w
+// and w + 4 of the 8 panels of 8 columns, which it factors in turn: the CPU
+// algorithm of func.cpp:806-836 in right-looking order -- pivot f (|f| and
+// a count if f < 0), column scaled by 1/sqrt(f) -- with the updates inside
+// a panel taken from the owning lanes by readlane.  A finished panel goes
+// to LP in panel order, LP[p][row][q] = L[row][8p + q], and the other
+// wavefronts apply it to their columns (rank-8 updates that read the 8
+// entries of a column as broadcast 16-byte LDS loads).  With the panels
+// dealt round-robin, the owner of the next panel has only that panel's 8
+// columns to bring up to date before it pivots: that catch-up and the
+// pivots are the critical path.  The wavefronts with time left build
+// X = inv(L) from 16x16 blocks: the diagonal blocks from 8x8 substitutions,
+// the off-diagonal blocks on the matrix cores,
 //   X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj.
 // ---------------------------------------------------------------------------
 
@@ -518,14 +518,6 @@ __device__ __forceinline__ void put_block(double *D, int pitch, dbl4 acc, double
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
 }
-// X_ij = -X_ii S for S = sum_{k=j}^{i-1} L_ik X_kj (S: this wavefront's scratch)
-__device__ __forceinline__ void x_block(const double *LP, double *X, double *S, int i, int j) {
-    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k = j; k < i; k++) acc = mm_lx(acc, LP, X, i, k, j);
-    put_block(S, 17, acc, 1.0);
-    const dbl4 z = {0.0, 0.0, 0.0, 0.0};
-    put_block(X + 16 * i * LDP + 16 * j, LDP, mm_xs(z, X, S, i), -1.0);
-}
 // S = sum_{k=j}^{i-1} L_ik X_kj (the sum of X_ij = -X_ii S without X_ii)
 __device__ __forceinline__ void x_sum(const double *LP, const double *X, double *S, int i, int j) {
     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -565,75 +557,97 @@ __device__ __forceinline__ void store_xrows(const double *X, double *Linv, int r
     }
 }
 
-// This lane's 16 columns of row d0 + lane of L to global memory (zeros above
-// the diagonal: a lane above a pivot column takes 0 at that pivot).
-__device__ __forceinline__ void store_lrow(const double (&g)[16], double *row) {
+// Columns 8G..8G+7 of this lane's g (group G = 0 or 1) to row[0..7]: 8
+// columns of row d0 + lane of L (zeros above the diagonal: a lane above a
+// pivot column takes 0 at that pivot)
+__device__ __forceinline__ void store_lrow8(const double (&g)[16], int G, double *row) {
 #pragma unroll
-    for (int c = 0; c < 16; c += 2) {
+    for (int c = 0; c < 8; c += 2) {
         dbl2 v;
-        v.x = g[c];
-        v.y = g[c + 1];
+        v.x = g[8 * G + c];
+        v.y = g[8 * G + c + 1];
         *reinterpret_cast<dbl2 *>(row + c) = v;
     }
 }
 
 // One panel step of the factor (all four wavefronts; ends with a barrier).
-// Wavefronts without columns left build inv(L) meanwhile, each piece as
-// soon as its inputs are final: X_00 (step 2), X_11 (4), X_10 (5), X_22 (6),
-// the sums S_20, S_21 of X_20, X_21 and the k < 2 terms of S_30, S_31 (6),
-// X_21 = -X_22 S_21, X_20 = -X_22 S_20 and S_32 = L_32 X_22 (7);
-// rows 0-15 (4) and 16-31 (6) of inv(L) to global memory.  S: the
-// wavefront's 16x17 scratch, S3: the three partial sums S_3j.
+// Wavefront w owns panels w and w + 4 (columns 8w.. in g[0..7], 32 + 8w..
+// in g[8..15]).  The owner of panel P applies panel P-1 to the panel's 8
+// columns (the earlier panels are already in them), pivots it and stores
+// its 8 columns of L.  The other wavefronts apply panel P-1 to their columns
+// not yet pivoted; the owner of the previous step, which skipped its other
+// group while pivoting, applies panel P-2 to that group first, so every
+// column receives the panels in increasing order.  Beside this, the
+// wavefronts with time left build inv(L), each piece as soon as its inputs
+// are final: X_00 (step 2), S_10 = L_10 X_00 (3), X_11 (4), X_10 = -X_11
+// S_10, S_21 = L_21 X_11 and the k = 1 term of S_31 (5), X_22, S_20 and the
+// k < 2 terms of S_30 (6), X_20, X_21 and S_32 = L_32 X_22 (7); rows 0-15
+// (5) and 16-31 (7) of inv(L) go to global memory.  S: the wavefront's
+// 16x17 scratch (the four lie SW apart, and a sum left in one wavefront's
+// scratch is read by another in a later step), S3: the partial sums S_3j.
 template <int P>
 __device__ __forceinline__ void factor_step(double (&g)[16], double *LP, double *ivs, double *X, double *S,
                                             double *S3, double &myinv, int &nneg, int w, double *lrow,
                                             double *Linv) {
-    constexpr int O = P >> 1, LOC = P & 1;
+    constexpr int O = P & 3, LOC = P >> 2, SW = 16 * 17;
     if (w == O) {
-        if constexpr (LOC == 0 && P > 0) {
-            double x[8];
-            panel_row<P - 1>(LP, x);
-            __builtin_amdgcn_sched_barrier(0);
-            panel_update16<0, 16>(g, x, LP + (P - 1) * (TB * 8), 16 * w);
-        }
-        pivot_panel<P, 8 * LOC>(g, LP, ivs, myinv, nneg);
-        // the owner's second panel completes its columns of L: stored now,
-        // beside the steps that follow
-        if constexpr (LOC == 1) store_lrow(g, lrow);
-        if constexpr (LOC == 0) {
-            double x[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) x[q] = g[q];
-            __builtin_amdgcn_sched_barrier(0);
-            panel_update16<8, 8>(g, x, LP + P * (TB * 8), 16 * w);
-        }
-    } else if (w > O) {
         if constexpr (P > 0) {
             double x[8];
             panel_row<P - 1>(LP, x);
             __builtin_amdgcn_sched_barrier(0);
-            panel_update16<0, 16>(g, x, LP + (P - 1) * (TB * 8), 16 * w);
+            panel_update16<8 * LOC, 8>(g, x, LP + (P - 1) * (TB * 8), 8 * P - 8 * LOC);
         }
-    } else if (LOC == 0 && w == O - 1) {
-        diag_block_inverse(LP, ivs, X, S, w);
-    } else if (P == 4 && w == 0) {
-        store_xrows(X, Linv, 0);
-    } else if (P == 5 && w == 0) {
-        x_block(LP, X, S, 1, 0);
-    } else if (P == 6) {
-        // S = T + 64 + w*272: wavefront 0's scratch takes S_20, wavefront 1's S_21
-        if (w == 0) {
-            store_xrows(X, Linv, 16);
-            x_sum(LP, X, S, 2, 0);
-        } else {
-            x_sum(LP, X, S, 2, 1);
-            s3_part(LP, X, S3, 0, 0, 2, false);
-            s3_part(LP, X, S3 + 16 * 17, 1, 1, 2, false);
+        pivot_panel<P, 8 * LOC>(g, LP, ivs, myinv, nneg);
+        store_lrow8(g, LOC, lrow + 8 * P);
+    } else {
+        if constexpr (P > 0) {
+            double x[8];
+            panel_row<P - 1>(LP, x);
+            __builtin_amdgcn_sched_barrier(0);
+            if (w > P) panel_update16<0, 8>(g, x, LP + (P - 1) * (TB * 8), 8 * w);
+            if (w + 4 > P) {
+                if constexpr (P >= 2) {
+                    if (w == P - 1) {
+                        double x2[8];
+                        panel_row<P - 2>(LP, x2);
+                        __builtin_amdgcn_sched_barrier(0);
+                        panel_update16<8, 8>(g, x2, LP + (P - 2) * (TB * 8), 8 * w + 24);
+                    }
+                }
+                panel_update16<8, 8>(g, x, LP + (P - 1) * (TB * 8), 8 * w + 24);
+            }
         }
-    } else if (P == 7) {
-        if (w == 0) s3_part(LP, X, S3 + 2 * (16 * 17), 2, 2, 3, false);
-        else if (w == 1) x_final(X, S, 2, 1);
-        else x_final(X, S - 2 * (16 * 17), 2, 0);
+        if constexpr (P == 2) {
+            if (w == 0) diag_block_inverse(LP, ivs, X, S, 0);
+        } else if constexpr (P == 3) {
+            if (w == 1) x_sum(LP, X, S, 1, 0);
+        } else if constexpr (P == 4) {
+            if (w == 2) diag_block_inverse(LP, ivs, X, S, 1);
+        } else if constexpr (P == 5) {
+            if (w == 0) {
+                x_final(X, S + SW, 1, 0);
+                store_xrows(X, Linv, 0);
+            } else if (w == 2) {
+                x_sum(LP, X, S, 2, 1);
+                s3_part(LP, X, S3 + SW, 1, 1, 2, false);
+            }
+        } else if constexpr (P == 6) {
+            if (w == 0) {
+                diag_block_inverse(LP, ivs, X, S, 2);
+            } else if (w == 1) {
+                x_sum(LP, X, S, 2, 0);
+                s3_part(LP, X, S3, 0, 0, 2, false);
+            }
+        } else if constexpr (P == 7) {
+            if (w == 0) {
+                x_final(X, S + SW, 2, 0);
+                store_xrows(X, Linv, 16);
+            } else if (w == 1) {
+                x_final(X, S + SW, 2, 1);
+            } else {
+                s3_part(LP, X, S3 + 2 * SW, 2, 2, 3, false);
+            }
+        }
     }
     __syncthreads();
 }
@@ -647,10 +661,13 @@ __device__ __forceinline__ void diag_factor_inv(double *T, double *LP, double *X
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     double g[16];
 #pragma unroll
-    for (int c = 0; c < 16; c++) g[c] = T[lane * LDP + 16 * w + c];
+    for (int c = 0; c < 8; c++) {
+        g[c] = T[lane * LDP + 8 * w + c];
+        g[8 + c] = T[lane * LDP + 32 + 8 * w + c];
+    }
     __syncthreads();
     double *ivs = T, *S = T + TB + w * (16 * 17), *S3 = T + TB + 4 * (16 * 17);
-    double *lrow = A + (size_t)(d0 + lane) * ld + d0 + 16 * w;
+    double *lrow = A + (size_t)(d0 + lane) * ld + d0;
     int nneg = 0;
     double myinv = 0.0;
     factor_step<0>(g, LP, ivs, X, S, S3, myinv, nneg, w, lrow, Linv);
