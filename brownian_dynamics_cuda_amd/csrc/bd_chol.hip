@@ -682,17 +682,6 @@ __device__ __forceinline__ void diag_factor_inv(double *T, double *LP, double *X
     // X_33, the k = 2 terms of S_30 and S_31, and rows 32-47 of inv(L)
     if (w == 3) {
         diag_block_inverse(LP, ivs, X, S, 3);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        // X_33 to rows 48-63, columns 48-63 of inv(L)
-#pragma unroll
-        for (int e = lane; e < 16 * 8; e += 64) {
-            const int r = 48 + e / 8, c = 48 + 2 * (e % 8);
-            dbl2 v;
-            v.x = X[r * LDP + c];
-            v.y = X[r * LDP + c + 1];
-            *reinterpret_cast<dbl2 *>(Linv + r * TB + c) = v;
-        }
     } else if (w == 0) {
         s3_part(LP, X, S3, 0, 2, 3, true);
     } else if (w == 1) {
@@ -710,6 +699,16 @@ __device__ __forceinline__ void diag_factor_inv(double *T, double *LP, double *X
         const int i16 = lane & 15, k4 = lane >> 4;
 #pragma unroll
         for (int v = 0; v < 4; v++) Linv[(48 + k4 + 4 * v) * TB + 16 * w + i16] = -x3[v];
+    } else {
+        // X_33 to rows 48-63, columns 48-63 of inv(L)
+#pragma unroll
+        for (int e = lane; e < 16 * 8; e += 64) {
+            const int r = 48 + e / 8, c = 48 + 2 * (e % 8);
+            dbl2 v;
+            v.x = X[r * LDP + c];
+            v.y = X[r * LDP + c + 1];
+            *reinterpret_cast<dbl2 *>(Linv + r * TB + c) = v;
+        }
     }
 }
 
