@@ -133,6 +133,42 @@ __device__ __forceinline__ double exp_neg_c(double x, const C &k) {
     return __builtin_ldexp(p, (int)n);
 }
 __device__ __forceinline__ double exp_neg(double x) { return exp_neg_c(x, kExpCoef); }
+// 2^(j/32) for j = 0..31, each the double nearest to the exact value
+struct Exp2Tab {
+    double t[32];
+};
+constexpr Exp2Tab kExp2Tab = {{
+                              1.0, 1.0218971486541166, 1.0442737824274138, 1.0671404006768237,
+                              1.0905077326652577, 1.1143867425958924, 1.1387886347566916, 1.1637248587775775,
+                              1.189207115002721, 1.215247359980469, 1.241857812073484, 1.2690509571917332,
+                              1.2968395546510096, 1.3252366431597413, 1.3542555469368927, 1.383909881963832,
+                              1.4142135623730951, 1.4451808069770467, 1.4768261459394993, 1.5091644275934228,
+                              1.5422108254079407, 1.5759808451078865, 1.6104903319492543, 1.645755478153965,
+                              1.681792830507429, 1.718619298122478, 1.7562521603732995, 1.7947090750031072,
+                              1.8340080864093424, 1.8741676341103, 1.9152065613971474, 1.9571441241754002}};
+// Taylor coefficients of e^r from r^6 down to r^2: 1/6!, 1/5!, 1/4!, 1/3!, 1/2
+struct ExpCoef6 {
+    double c[5];
+};
+constexpr ExpCoef6 kExpCoef6 = {{1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0, 0.5}};
+// e^x for -700 < x <= 0 with a 32-entry table (Tang's method): x = (32n + j)
+// ln2/32 + r with m = 32n + j = rint(32x/ln2) and |r| <= ln2/64, the two-part
+// ln2 above scaled by 1/32 (so m ln2_hi/32 is exact); e^r by its Taylor
+// series to r^6 (truncation below 4e-18), times tab[j] = 2^(j/32) and 2^n.
+// Seven FMAs instead of the twelve of exp_neg_c, for one table read.
+template <typename C>
+__host__ __device__ __forceinline__ double exp_neg_t(double x, const double *tab, const C &k) {
+    const double m = rint(x * 46.166241308446828);
+    double r = __builtin_fma(-m, 6.93147180369123816490e-01 / 32.0, x);
+    r = __builtin_fma(-m, 1.90821492927058770002e-10 / 32.0, r);
+    const int im = (int)m;
+    double p = k.c[0];
+#pragma unroll
+    for (int i = 1; i < 5; i++) p = __builtin_fma(p, r, k.c[i]);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    return __builtin_ldexp(tab[im & 31] * p, im >> 5);
+}
 // sin and cos of x for |x| < 2^19 (the frame angles here lie within
 // [-2 pi, 2 pi]), branch-free: x = n pi/2 + r with n = rint(2x/pi) and pi/2
 // in three parts (the published constants of fdlibm's e_rem_pio2.c: two

@@ -969,14 +969,20 @@ constexpr int kCCThreads = 256;   // four wavefronts, a work unit each
 // summed in world axes about its centre and turned into body components by
 // k_cc_gather (equal to the reference's body-frame sum for an orthonormal
 // frame).  Lanes past the last charge repeat it and are weighted by 0.
-// The Taylor coefficients of e^r come in as a kernel argument, so they sit
-// in scalar registers and each Horner step is one FMA with a scalar addend
-// (compile-time constants end up in vector registers, where every step
-// first copies its addend).
+// exp(-kappa d) is the table form exp_neg_t: 2^(j/32) comes in as a kernel
+// argument and is copied to LDS once (32 doubles over the 64 banks, so any
+// mix of lanes reads it without conflict); the Taylor coefficients of e^r
+// come in as a kernel argument too, so they sit in scalar registers and each
+// Horner step is one FMA with a scalar addend (compile-time constants end up
+// in vector registers, where every step first copies its addend).
 template <bool WITH_E>
 __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, const double *__restrict__ Qp,
                                                        const int *__restrict__ list, const int *__restrict__ count,
-                                                       int nkb, int nlc, double *__restrict__ slots, ExpCoef ec) {
+                                                       int nkb, int nlc, double *__restrict__ slots, ExpCoef6 ec,
+                                                       Exp2Tab et) {
+    __shared__ double etab[32];
+    if (threadIdx.x < 32) etab[threadIdx.x] = et.t[threadIdx.x];
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nsub = nkb * nlc, nunits = (*count) * nsub, Nq = s.Nq;
@@ -1004,7 +1010,7 @@ __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, const dou
             const double r2s = dot(z, z);
             const double inv = rsqrt_f64(r2s);
             const double dist = r2s * inv;
-            const double t = keq * d[3] * exp_neg_c(-kappa * dist, ec);
+            const double t = keq * d[3] * exp_neg_t(-kappa * dist, etab, ec);
             if (WITH_E) E += t * inv;
             double g1d = t * (kappa * dist + 1.0) * (inv * inv * inv);
             if (dist <= 8.0) {
@@ -1093,10 +1099,10 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
         M(K_CCWORK, 0);
         if (Epart)
             k_cc_work<true><<<w.cc_grid, kCCThreads, 0, qq>>>(s, g, st.Qp, w.cc_list, w.cc_count, w.cc_nkb,
-                                                              w.cc_nlc, w.cc_slots, kExpCoef);
+                                                              w.cc_nlc, w.cc_slots, kExpCoef6, kExp2Tab);
         else
             k_cc_work<false><<<w.cc_grid, kCCThreads, 0, qq>>>(s, g, st.Qp, w.cc_list, w.cc_count, w.cc_nkb,
-                                                               w.cc_nlc, w.cc_slots, kExpCoef);
+                                                               w.cc_nlc, w.cc_slots, kExpCoef6, kExp2Tab);
         M(K_CCWORK, 1);
     };
     auto mech = [&]() { M(K_MECH, 0); k_mech<<<nblk(s.n, 64), 64, 0, q>>>(s, g, f, E(off_mech), terms); M(K_MECH, 1); };
