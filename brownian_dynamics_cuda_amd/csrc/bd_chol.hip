@@ -871,8 +871,22 @@ __global__ void __launch_bounds__(256) k_noise(const double *A, int ld, int m, i
         return;
     }
     const int ib = row / TB, r = row % TB;
+    const double *a = A + (size_t)r * ld + ib * TB + lane;  // parked tile (t, ib): a + t*TB*ld
+    const double *q = p + lane;
     double v = 0.0;
-    for (int t = 0; t < ib; t++) v += A[(size_t)(t * TB + r) * ld + ib * TB + lane] * p[t * TB + lane];
+    int t = 0;
+    // eight tiles' loads in flight at a time, summed in tile order
+    for (; t + 8 <= ib; t += 8) {
+        double x[8], y[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            x[u] = a[(size_t)(t + u) * TB * ld];
+            y[u] = q[(t + u) * TB];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) v += x[u] * y[u];
+    }
+    for (; t < ib; t++) v += a[(size_t)t * TB * ld] * q[t * TB];
     if (lane <= r) v += A[(size_t)row * ld + ib * TB + lane] * p[ib * TB + lane];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     if (lane == 0) rd[row] = s2dt * v;
