@@ -214,9 +214,17 @@ def main():
         for s in range(args.cpu_steps):
             orc.step(fs, rr[s], p[s])
         dc = time.perf_counter() - t1
+        model = "unknown"
+        try:
+            with open("/proc/cpuinfo") as fcpu:
+                model = next((ln.split(":", 1)[1].strip() for ln in fcpu if ln.startswith("model name")),
+                             model)
+        except OSError:
+            pass
         cpu = dict(value=args.cpu_steps / dc, unit="steps/s", cores=1, kind=kind,
                    sample=f"{args.cpu_steps} steps of the same fiber, noise and start state "
-                          f"({dc:.1f} s), one thread")
+                          f"({dc:.1f} s), one thread",
+                   host=dict(cpu=model, threads_available=len(os.sched_getaffinity(0))))
 
     out = {
         "metric": "BD steps/s, 100-nucleosome fiber: FP64 vs mixed Cholesky at <=1e-6 rel err",
