@@ -240,15 +240,17 @@ __device__ __forceinline__ bool dhev_recv(double q1q2, double c4pe, double kappa
                                           double &E) {
     double3 dv = p - q;
     double d2 = dot(dv, dv);
-    if (!(d2 < 49.01)) return false;
-    double d = sqrt(d2);
-    if (!(d < 7.0)) return false;  // beyond both cut-offs: exactly zero
+    // The reference's cut-offs d < 7 and d < 4 on d = sqrt(d2), tested on d2:
+    // sqrt is correctly rounded and monotonic, 49 and 16 are exact squares,
+    // and the double below each has its root below 7 (4), so sqrt(d2) < 7
+    // exactly when d2 < 49 (and < 4 when d2 < 16).  Beyond: exactly zero.
+    if (!(d2 < 49.0)) return false;
     // the reference's quotients by d, d^2, d^7 and d^13 as products with
-    // one reciprocal of d
-    const double id = 1.0 / d, qe = q1q2 / c4pe * exp_neg(-kappa * d);
+    // 1/d from the refined reciprocal square root (no IEEE root or division)
+    const double id = rsqrt_f64(d2), d = d2 * id, qe = q1q2 / c4pe * exp_neg(-kappa * d);
     double fs = qe * (kappa * d + 1) * (id * id);
     E += qe * id;
-    if (d < 4.0) {
+    if (d2 < 16.0) {
         const double x = sigma * id, x2 = x * x, x6 = x2 * x2 * x2;
         const double s6 = sigma * sigma, s6b = s6 * s6 * s6;
         const double i2 = id * id, i7 = i2 * i2 * i2 * id;  // d^-7
@@ -267,10 +269,8 @@ __device__ __forceinline__ bool dh_recv(double q1q2, double c4pe, double kappa, 
                                         double3 q, double3 &f, double &E) {
     double3 dv = p - q;
     double d2 = dot(dv, dv);
-    if (!(d2 < 49.01)) return false;
-    double d = sqrt(d2);
-    if (!(d < 7.0)) return false;
-    const double id = 1.0 / d, qe = q1q2 / c4pe * exp_neg(-kappa * d);
+    if (!(d2 < 49.0)) return false;  // d < 7, as in dhev_recv
+    const double id = rsqrt_f64(d2), d = d2 * id, qe = q1q2 / c4pe * exp_neg(-kappa * d);
     const double fd = qe * (kappa * d + 1) * (id * id * id);
     E += qe * id;
     f.x += clamp09(fd * dv.x);
@@ -284,10 +284,8 @@ __device__ __forceinline__ bool ev_recv(double kev, double sigma, double3 p, dou
                                         double3 &f, double &E) {
     double3 dv = p - q;
     double d2 = dot(dv, dv);
-    if (!(d2 < 16.01)) return false;
-    double d = sqrt(d2);
-    if (!(d < 4.0)) return false;
-    const double id = 1.0 / d;
+    if (!(d2 < 16.0)) return false;  // d < 4, as in dhev_recv
+    const double id = rsqrt_f64(d2);
     const double x = sigma * id, x2 = x * x, x6 = x2 * x2 * x2;
     const double s6 = sigma * sigma, s6b = s6 * s6 * s6;
     const double i2 = id * id, i7 = i2 * i2 * i2 * id;  // d^-7
