@@ -264,6 +264,12 @@ __device__ __forceinline__ bool dhev_recv(double q1q2, double c4pe, double kappa
     return true;
 }
 
+// Squared forms of the reference's other cut-offs on d = sqrt(d2) (the root
+// correctly rounded, so monotonic): d <= 8 exactly when d2 <= kD8Sq, the
+// double after 64 (its root rounds down to 8); d > 25 exactly when d2 > 625.
+constexpr double kD8Sq = 0x1.0000000000001p+6;
+constexpr double kD25Sq = 625.0;
+
 // Force_Electrostatics: DH below 7 nm only.
 __device__ __forceinline__ bool dh_recv(double q1q2, double c4pe, double kappa, double3 p,
                                         double3 q, double3 &f, double &E) {

@@ -320,14 +320,15 @@ __global__ void k_mech(Stat s, Geo g, Frc f, double *Epart, int terms) {
 // ---------------------------------------------------------------------------
 // core / linker nonbonded helpers (func.cpp:1291-1605, no clamps, EV <= 8 nm)
 // ---------------------------------------------------------------------------
-// scalar g1 of the reference for a charge pair at distance d; adds energy
-__device__ __forceinline__ double dh_core(double ke_q1q2, double kappa, double d, double &E) {
-    const double id = 1.0 / d, qe = ke_q1q2 * exp_neg(-kappa * d);
+// scalar g1 of the reference for a charge pair at distance d (id = 1/d);
+// adds energy
+__device__ __forceinline__ double dh_core(double ke_q1q2, double kappa, double d, double id, double &E) {
+    const double qe = ke_q1q2 * exp_neg(-kappa * d);
     E += qe * id;
     return qe * (kappa * d + 1.0) * (id * id);
 }
-__device__ __forceinline__ double ev_core(double kex, double sg, double d, double &E) {
-    double x = sg / d, x2 = x * x, x6 = x2 * x2 * x2, x7 = x6 * x, x13 = x6 * x7;
+__device__ __forceinline__ double ev_core(double kex, double sg, double id, double &E) {
+    double x = sg * id, x2 = x * x, x6 = x2 * x2 * x2, x7 = x6 * x, x13 = x6 * x7;
     E += kex * (x6 * x6 - x6);
     return kex * ((12.0 / sg) * x13 - (6.0 / sg) * x7);
 }
@@ -359,14 +360,14 @@ __global__ void __launch_bounds__(kLinkerThreads) k_linker(Stat s, Geo g, Sites 
             int i = s.lnk[pi];
             if (!s.elig[i] || abs(i - j) <= 1) continue;
             double3 dv = rj - ld3(g.r + 3 * i);
-            double dist = sqrt(dot(dv, dv));
-            if (dist > 25.0) continue;
+            const double d2 = dot(dv, dv);
+            if (d2 > kD25Sq) continue;  // dist > 25
             double Ep = 0.0;
-            const double id = 1.0 / dist, qe = s.k_e * qlql * exp_neg(-kll * dist);
+            const double id = rsqrt_f64(d2), dist = d2 * id, qe = s.k_e * qlql * exp_neg(-kll * dist);
             double g1 = qe * (kll * dist + 1.0) * (id * id);
             Ep += qe * id;
-            if (dist <= 8) g1 += ev_core(s.k_ex, s.sig_dd, dist, Ep);
-            F += (g1 / dist) * dv;
+            if (d2 <= kD8Sq) g1 += ev_core(s.k_ex, s.sig_dd, id, Ep);  // dist <= 8
+            F += (g1 * id) * dv;
             if (i > j) E += Ep;
         }
         // DNA-core: every charge of the eligible cores within 25 nm, the
@@ -389,12 +390,12 @@ __global__ void __launch_bounds__(kLinkerThreads) k_linker(Stat s, Geo g, Sites 
                 const double *Q = st.Q + 3 * (size_t)kc * s.Nq;
                 for (int q = tid; q < s.Nq; q += nth) {
                     double3 z = rj - ld3(Q + 3 * q);
-                    double d = sqrt(dot(z, z));
+                    const double d2 = dot(z, z), id = rsqrt_f64(d2), d = d2 * id;
                     double cq = s.cq[q];
                     double g1 = 0.0;
-                    if (dh) g1 += dh_core(s.k_e * s.q_l * cq, s.debye, d, Ed);
-                    if (d <= 8.0 && cq > 0) g1 += ev_core(s.k_ex, s.sig_dc, d, Ed);
-                    F += (g1 / d) * z;
+                    if (dh) g1 += dh_core(s.k_e * s.q_l * cq, s.debye, d, id, Ed);
+                    if (d2 <= kD8Sq && cq > 0) g1 += ev_core(s.k_ex, s.sig_dc, id, Ed);  // d <= 8
+                    F += (g1 * id) * z;
                 }
             }
             __syncthreads();
@@ -650,11 +651,11 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
                         const double cq = s.cq[q];
                         for (int e = sub; e < cnt; e += S) {
                             const double3 z = Qq - ld3(s_pos + 3 * e);
-                            const double d = sqrt(dot(z, z));
+                            const double d2 = dot(z, z), id = rsqrt_f64(d2), d = d2 * id;
                             double g1 = 0.0;
-                            if (abs(cb - s_list[e]) > 1) g1 += dh_core(s.k_e * s.q_l * cq, s.debye, d, E);
-                            if (d <= 8.0 && cq > 0) g1 += ev_core(s.k_ex, s.sig_dc, d, E);
-                            Fq[i] += (g1 / d) * z;
+                            if (abs(cb - s_list[e]) > 1) g1 += dh_core(s.k_e * s.q_l * cq, s.debye, d, id, E);
+                            if (d2 <= kD8Sq && cq > 0) g1 += ev_core(s.k_ex, s.sig_dc, id, E);  // d <= 8
+                            Fq[i] += (g1 * id) * z;
                         }
                     }
                 }
@@ -1013,7 +1014,7 @@ __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, const dou
             const double t = keq * d[3] * exp_neg_t(-kappa * dist, etab, ec);
             if (WITH_E) E += t * inv;
             double g1d = t * (kappa * dist + 1.0) * (inv * inv * inv);
-            if (dist <= 8.0) {
+            if (r2s <= kD8Sq) {  // the reference's d <= 8, exactly
                 const double x = sg * inv, x2 = x * x, x6 = x2 * x2 * x2, x7 = x6 * x, x13 = x6 * x7;
                 if (WITH_E) E += kex * (x6 * x6 - x6);
                 g1d += kex * (c12 * x13 - c6 * x7) * inv;
