@@ -246,8 +246,10 @@ __device__ __forceinline__ bool dhev_recv(double q1q2, double c4pe, double kappa
     // exactly when d2 < 49 (and < 4 when d2 < 16).  Beyond: exactly zero.
     if (!(d2 < 49.0)) return false;
     // the reference's quotients by d, d^2, d^7 and d^13 as products with
-    // 1/d from the refined reciprocal square root (no IEEE root or division)
-    const double id = rsqrt_f64(d2), d = d2 * id, qe = q1q2 / c4pe * exp_neg(-kappa * d);
+    // 1/d from the refined reciprocal square root (no IEEE root or division),
+    // and its quotient by 4 pi eps as a product with the reciprocal, which
+    // does not change along a partner loop
+    const double id = rsqrt_f64(d2), d = d2 * id, qe = q1q2 * (1.0 / c4pe) * exp_neg(-kappa * d);
     double fs = qe * (kappa * d + 1) * (id * id);
     E += qe * id;
     if (d2 < 16.0) {
@@ -276,7 +278,7 @@ __device__ __forceinline__ bool dh_recv(double q1q2, double c4pe, double kappa, 
     double3 dv = p - q;
     double d2 = dot(dv, dv);
     if (!(d2 < 49.0)) return false;  // d < 7, as in dhev_recv
-    const double id = rsqrt_f64(d2), d = d2 * id, qe = q1q2 / c4pe * exp_neg(-kappa * d);
+    const double id = rsqrt_f64(d2), d = d2 * id, qe = q1q2 * (1.0 / c4pe) * exp_neg(-kappa * d);
     const double fd = qe * (kappa * d + 1) * (id * id * id);
     E += qe * id;
     f.x += clamp09(fd * dv.x);
