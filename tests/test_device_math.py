@@ -7,6 +7,8 @@ host pass and measured against 80-bit long double references:
   against the correctly rounded values from exact decimal arithmetic;
 - sincos_small (the sin/cos of the linker mechanics and the rotation,
   func.cpp:951-1289 and 1607-1670).
+The squared forms of the reference's distance cut-offs are checked against
+numpy's correctly rounded square root on the doubles around each threshold.
 """
 import os
 import shutil
@@ -91,3 +93,30 @@ def test_exp_neg_t_accuracy(measured):
 def test_sincos_small_accuracy(measured):
     assert measured["sin"] <= 3e-16
     assert measured["cos"] <= 3e-16
+
+
+def _doubles_around(t, n=200000):
+    import numpy as np
+    base = np.array([t], dtype=np.float64).view(np.int64)[0]
+    return (base + np.arange(-n, n + 1, dtype=np.int64)).view(np.float64)
+
+
+def test_squared_cutoffs_exact():
+    """The kernels test the reference's cut-offs on d = sqrt(d2) as tests on d2
+    (func.cpp:278-371 for 7 and 4 nm; 1291-1605 for 8 and 25 nm): each squared
+    threshold must give the same answer as the correctly rounded root for every
+    double near it, and bd_dev.h must hold exactly these thresholds."""
+    import re
+    import numpy as np
+    src = open(os.path.join(CSRC, "bd_dev.h")).read()
+    d8 = float.fromhex(re.search(r"kD8Sq = (0x[0-9a-fA-Fp.+]+);", src).group(1))
+    d25 = float(re.search(r"kD25Sq = ([0-9.]+);", src).group(1))
+    assert d8 == np.nextafter(64.0, np.inf) and d25 == 625.0
+    assert "if (!(d2 < 49.0)) return false;" in src and "if (d2 < 16.0) {" in src
+    for r, t, strict in ((7.0, 49.0, True), (4.0, 16.0, True), (8.0, d8, False), (25.0, d25, False)):
+        x = _doubles_around(t)
+        root = np.sqrt(x)
+        if strict:
+            assert np.array_equal(root < r, x < t), r
+        else:
+            assert np.array_equal(root <= r, x <= t), r
