@@ -46,6 +46,11 @@ struct Frc {
 // Static device data (everything the step reads but never writes).
 struct Stat {
     int n, n_c, n_tail, n_t, n_lh_n, n_lh_g, n_lh_c, nlh, n_LH, n_lhc, Nq, n_lnk;
+    // the Taylor coefficients of exp_neg (kExpCoef), carried in the kernel
+    // argument so that the pair kernels read them into scalar registers
+    struct {
+        double c[11];
+    } ec;
     const int *type;      // n
     const int *ic_of;     // n: ordinal of the last core at or before bead i
     const int *core_bead; // n_c: bead index of core k
@@ -235,7 +240,8 @@ __device__ __forceinline__ double clamp09(double v) {
 // Pairs beyond both cut-offs contribute exactly zero and are skipped.
 
 // Force_Ele_Vdw: DH below 7 nm plus excluded volume below 4 nm.
-__device__ __forceinline__ bool dhev_recv(double q1q2, double c4pe, double kappa, double kev,
+template <typename EC>
+__device__ __forceinline__ bool dhev_recv(const EC &ec, double q1q2, double c4pe, double kappa, double kev,
                                           double sigma, double3 p, double3 q, double3 &f,
                                           double &E) {
     double3 dv = p - q;
@@ -249,7 +255,7 @@ __device__ __forceinline__ bool dhev_recv(double q1q2, double c4pe, double kappa
     // 1/d from the refined reciprocal square root (no IEEE root or division),
     // and its quotient by 4 pi eps as a product with the reciprocal, which
     // does not change along a partner loop
-    const double id = rsqrt_f64(d2), d = d2 * id, qe = q1q2 * (1.0 / c4pe) * exp_neg(-kappa * d);
+    const double id = rsqrt_f64(d2), d = d2 * id, qe = q1q2 * (1.0 / c4pe) * exp_neg_c(-kappa * d, ec);
     double fs = qe * (kappa * d + 1) * (id * id);
     E += qe * id;
     if (d2 < 16.0) {
@@ -273,12 +279,13 @@ constexpr double kD8Sq = 0x1.0000000000001p+6;
 constexpr double kD25Sq = 625.0;
 
 // Force_Electrostatics: DH below 7 nm only.
-__device__ __forceinline__ bool dh_recv(double q1q2, double c4pe, double kappa, double3 p,
+template <typename EC>
+__device__ __forceinline__ bool dh_recv(const EC &ec, double q1q2, double c4pe, double kappa, double3 p,
                                         double3 q, double3 &f, double &E) {
     double3 dv = p - q;
     double d2 = dot(dv, dv);
     if (!(d2 < 49.0)) return false;  // d < 7, as in dhev_recv
-    const double id = rsqrt_f64(d2), d = d2 * id, qe = q1q2 * (1.0 / c4pe) * exp_neg(-kappa * d);
+    const double id = rsqrt_f64(d2), d = d2 * id, qe = q1q2 * (1.0 / c4pe) * exp_neg_c(-kappa * d, ec);
     const double fd = qe * (kappa * d + 1) * (id * id * id);
     E += qe * id;
     f.x += clamp09(fd * dv.x);

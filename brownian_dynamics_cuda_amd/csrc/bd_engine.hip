@@ -422,6 +422,7 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     {
         double eps = 1 / (4 * kPI * sys->k_e);  // func.cpp:1874 passes 1/(4*PI*k_e)
         s.c4pe = 4 * kPI * eps;
+        for (int i = 0; i < 11; i++) s.ec.c[i] = kExpCoef.c[i];
     }
     s.rmax_q = rmax_q; s.rmax_g = rmax_g;
 

@@ -322,8 +322,10 @@ __global__ void k_mech(Stat s, Geo g, Frc f, double *Epart, int terms) {
 // ---------------------------------------------------------------------------
 // scalar g1 of the reference for a charge pair at distance d (id = 1/d);
 // adds energy
-__device__ __forceinline__ double dh_core(double ke_q1q2, double kappa, double d, double id, double &E) {
-    const double qe = ke_q1q2 * exp_neg(-kappa * d);
+template <typename EC>
+__device__ __forceinline__ double dh_core(const EC &ec, double ke_q1q2, double kappa, double d, double id,
+                                          double &E) {
+    const double qe = ke_q1q2 * exp_neg_c(-kappa * d, ec);
     E += qe * id;
     return qe * (kappa * d + 1.0) * (id * id);
 }
@@ -363,7 +365,7 @@ __global__ void __launch_bounds__(kLinkerThreads) k_linker(Stat s, Geo g, Sites 
             const double d2 = dot(dv, dv);
             if (d2 > kD25Sq) continue;  // dist > 25
             double Ep = 0.0;
-            const double id = rsqrt_f64(d2), dist = d2 * id, qe = s.k_e * qlql * exp_neg(-kll * dist);
+            const double id = rsqrt_f64(d2), dist = d2 * id, qe = s.k_e * qlql * exp_neg_c(-kll * dist, s.ec);
             double g1 = qe * (kll * dist + 1.0) * (id * id);
             Ep += qe * id;
             if (d2 <= kD8Sq) g1 += ev_core(s.k_ex, s.sig_dd, id, Ep);  // dist <= 8
@@ -393,7 +395,7 @@ __global__ void __launch_bounds__(kLinkerThreads) k_linker(Stat s, Geo g, Sites 
                     const double d2 = dot(z, z), id = rsqrt_f64(d2), d = d2 * id;
                     double cq = s.cq[q];
                     double g1 = 0.0;
-                    if (dh) g1 += dh_core(s.k_e * s.q_l * cq, s.debye, d, id, Ed);
+                    if (dh) g1 += dh_core(s.ec, s.k_e * s.q_l * cq, s.debye, d, id, Ed);
                     if (d2 <= kD8Sq && cq > 0) g1 += ev_core(s.k_ex, s.sig_dc, id, Ed);  // d <= 8
                     F += (g1 * id) * z;
                 }
@@ -403,19 +405,19 @@ __global__ void __launch_bounds__(kLinkerThreads) k_linker(Stat s, Geo g, Sites 
     }
     if (terms & TERM_TAIL)  // tail-linker, energy owned by tails
         for_near<kLinkerThreads>(st.ch_t, s.n_tail, rj, 0.0, tid, nth, s_list, s_wsum, [&](int t) {
-            dhev_recv(s.t_q[t] * s.q_l, s.c4pe, s.debye, s.k_ex, s.sig_tl, rj, ld3(g.r_t + 3 * t), F, Ed);
+            dhev_recv(s.ec, s.t_q[t] * s.q_l, s.c4pe, s.debye, s.k_ex, s.sig_tl, rj, ld3(g.r_t + 3 * t), F, Ed);
         });
     if (terms & TERM_LH) {
         // LH-DNA, energy owned by LH beads
         for_near<kLinkerThreads>(st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int i) {
             int idx = i % s.nlh;
             idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
-            dhev_recv(s.LH_q[idx] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[idx], rj, ld3(g.r_lh + 3 * i), F, Ed);
+            dhev_recv(s.ec, s.LH_q[idx] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[idx], rj, ld3(g.r_lh + 3 * i), F, Ed);
         });
         // LHG-DNA, energy on the core side
         for_near<kLinkerThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
             int m = x % s.n_lh_g;
-            dhev_recv(s.LH_q[m] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[m], rj, ld3(st.G + 3 * x), F, Ed);
+            dhev_recv(s.ec, s.LH_q[m] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[m], rj, ld3(st.G + 3 * x), F, Ed);
         });
     }
     double v[4] = {F.x, F.y, F.z, E};
@@ -469,11 +471,11 @@ __global__ void __launch_bounds__(kTailThreads) k_tail(Stat s, Geo g, Sites st, 
         for_near<kTailThreads>(st.ch_t, nt, rj, 0.0, tid, nth, s_list, s_wsum, [&](int k) {
             if (k == j) return;
             if (!(tg[k] != tg[j] || abs(k - j) > 2)) return;
-            dh_recv(qj * s.t_q[k], s.c4pe, s.debye, rj, ld3(g.r_t + 3 * k), F, (k > j) ? E : Ed);
+            dh_recv(s.ec, qj * s.t_q[k], s.c4pe, s.debye, rj, ld3(g.r_t + 3 * k), F, (k > j) ? E : Ed);
         });
         // tail-linker (1898-1917)
         for_near<kTailThreads>(st.ch_l, s.n_lnk, rj, 0.0, tid, nth, s_list, s_wsum, [&](int pi) {
-            dhev_recv(qj * s.q_l, s.c4pe, s.debye, s.k_ex, s.sig_tl, rj, ld3(g.r + 3 * s.lnk[pi]), F, E);
+            dhev_recv(s.ec, qj * s.q_l, s.c4pe, s.debye, s.k_ex, s.sig_tl, rj, ld3(g.r + 3 * s.lnk[pi]), F, E);
         });
     }
     if ((terms & TERM_LH) && s.t_fix[j] == 0) {
@@ -481,11 +483,11 @@ __global__ void __launch_bounds__(kTailThreads) k_tail(Stat s, Geo g, Sites st, 
         for_near<kTailThreads>(st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int i) {
             int idx = i % s.nlh;
             idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
-            dhev_recv(s.LH_q[idx] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rj, ld3(g.r_lh + 3 * i), F, Ed);
+            dhev_recv(s.ec, s.LH_q[idx] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rj, ld3(g.r_lh + 3 * i), F, Ed);
         });
         for_near<kTailThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
             int m = x % s.n_lh_g;
-            dhev_recv(s.LH_q[m] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], rj, ld3(st.G + 3 * x), F, Ed);
+            dhev_recv(s.ec, s.LH_q[m] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], rj, ld3(st.G + 3 * x), F, Ed);
         });
     }
     double v[4] = {F.x, F.y, F.z, E};
@@ -551,22 +553,22 @@ __global__ void __launch_bounds__(kLHThreads) k_lh(Stat s, Geo g, Sites st, Frc 
             if (k / nlh == mine) {
                 if (abs(k - j) > 1) ev_recv(s.k_ex, sg, rj, ld3(g.r_lh + 3 * k), F, Ek);
             } else {
-                dhev_recv(s.LH_q[idx] * s.LH_q[i2], s.c4pe, s.debye, s.k_ex, sg, rj, ld3(g.r_lh + 3 * k), F, Ek);
+                dhev_recv(s.ec, s.LH_q[idx] * s.LH_q[i2], s.c4pe, s.debye, s.k_ex, sg, rj, ld3(g.r_lh + 3 * k), F, Ek);
             }
         });
         // LH-DNA (2249-2279)
         for_near<kLHThreads>(st.ch_l, s.n_lnk, rj, 0.0, tid, nth, s_list, s_wsum, [&](int pi) {
-            dhev_recv(s.LH_q[idx] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[idx], rj, ld3(g.r + 3 * s.lnk[pi]), F, E);
+            dhev_recv(s.ec, s.LH_q[idx] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[idx], rj, ld3(g.r + 3 * s.lnk[pi]), F, E);
         });
         // LH-tail (2343-2398)
         for_near<kLHThreads>(st.ch_t, s.n_tail, rj, 0.0, tid, nth, s_list, s_wsum, [&](int t) {
-            dhev_recv(s.LH_q[idx] * s.t_q[t], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rj, ld3(g.r_t + 3 * t), F, E);
+            dhev_recv(s.ec, s.LH_q[idx] * s.t_q[t], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rj, ld3(g.r_t + 3 * t), F, E);
         });
         // LHG of other cores acting on this bead (2590-2644), energy on the core side
         for_near<kLHThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
             int k = x / s.n_lh_g, m = x % s.n_lh_g;
             if (k == mine) return;
-            dhev_recv(s.LH_q[m] * s.LH_q[idx], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[idx]) / 2, rj,
+            dhev_recv(s.ec, s.LH_q[m] * s.LH_q[idx], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[idx]) / 2, rj,
                       ld3(st.G + 3 * x), F, Ed);
         });
     }
@@ -653,7 +655,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
                             const double3 z = Qq - ld3(s_pos + 3 * e);
                             const double d2 = dot(z, z), id = rsqrt_f64(d2), d = d2 * id;
                             double g1 = 0.0;
-                            if (abs(cb - s_list[e]) > 1) g1 += dh_core(s.k_e * s.q_l * cq, s.debye, d, id, E);
+                            if (abs(cb - s_list[e]) > 1) g1 += dh_core(s.ec, s.k_e * s.q_l * cq, s.debye, d, id, E);
                             if (d2 <= kD8Sq && cq > 0) g1 += ev_core(s.k_ex, s.sig_dc, id, E);  // d <= 8
                             Fq[i] += (g1 * id) * z;
                         }
@@ -684,7 +686,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
                         for (int e = sub; e < cnt; e += S) {
                             const int k2 = s_list[e];
                             for (int m = 0; m < s.n_lh_g; m++)
-                                dhev_recv(s.LH_q[m] * cq, s.c4pe, s.debye, s.k_ex, s.LH_hc[m], Qq,
+                                dhev_recv(s.ec, s.LH_q[m] * cq, s.c4pe, s.debye, s.k_ex, s.LH_hc[m], Qq,
                                           ld3(st.G + 3 * (k2 * s.n_lh_g + m)), Fq[i], Ed);
                         }
                     }
@@ -718,14 +720,14 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
             double3 rt = ld3(g.r_t + 3 * t);
 #pragma unroll
             for (int m = 0; m < kMaxGSites; m++) if (m < ng)
-                dhev_recv(s.LH_q[m] * s.t_q[t], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], Gm[m], rt, Fm[m], E);
+                dhev_recv(s.ec, s.LH_q[m] * s.t_q[t], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], Gm[m], rt, Fm[m], E);
         });
         // LHG-DNA (2481-2522)
         for_near<kCoreThreads>(st.ch_l, s.n_lnk, hc, hr, ft, FT, s_list, s_wsum, [&](int pi) {
             double3 rl = ld3(g.r + 3 * s.lnk[pi]);
 #pragma unroll
             for (int m = 0; m < kMaxGSites; m++) if (m < ng)
-                dhev_recv(s.LH_q[m] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[m], Gm[m], rl, Fm[m], E);
+                dhev_recv(s.ec, s.LH_q[m] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[m], Gm[m], rl, Fm[m], E);
         });
         // LHG-charges of every other core (2526-2586): the cores k = sub
         // (mod S) whose charges (within rmax_q of the core) can come within
@@ -747,7 +749,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
                     const double3 Qq = ld3(Qk + 3 * q);
 #pragma unroll
                     for (int m = 0; m < kMaxGSites; m++) if (m < ng)
-                        dhev_recv(s.LH_q[m] * s.cq[q], s.c4pe, s.debye, s.k_ex, s.LH_hc[m], Gm[m], Qq, Fm[m], E);
+                        dhev_recv(s.ec, s.LH_q[m] * s.cq[q], s.c4pe, s.debye, s.k_ex, s.LH_hc[m], Gm[m], Qq, Fm[m], E);
                 }
             }
             __syncthreads();
@@ -760,7 +762,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
             double3 rl = ld3(g.r_lh + 3 * jl);
 #pragma unroll
             for (int m = 0; m < kMaxGSites; m++) if (m < ng)
-                dhev_recv(s.LH_q[m] * s.LH_q[i2], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[i2]) / 2, Gm[m], rl, Fm[m], E);
+                dhev_recv(s.ec, s.LH_q[m] * s.LH_q[i2], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[i2]) / 2, Gm[m], rl, Fm[m], E);
         });
         // LHG-LHG with the other LH-bound cores (2649-2719)
         for_near<kCoreThreads>(st.ch_g, s.n_lhc * ng, hc, hr, ft, FT, s_list, s_wsum, [&](int x) {
@@ -769,7 +771,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
             double3 G2 = ld3(st.G + 3 * x);
 #pragma unroll
             for (int m = 0; m < kMaxGSites; m++) if (m < ng)
-                dhev_recv(s.LH_q[m] * s.LH_q[m1], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[m1]) / 2, Gm[m], G2, Fm[m],
+                dhev_recv(s.ec, s.LH_q[m] * s.LH_q[m1], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[m1]) / 2, Gm[m], G2, Fm[m],
                           (k > lho) ? E : Ed);
         });
         // head - first C-term bead spring (2727-2777): force x10, torque x1
@@ -818,14 +820,14 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
                 const double3 rl = ld3(g.r_lh + 3 * i);
 #pragma unroll
                 for (int e = 0; e < kMaxRT; e++)
-                    if (e < ne) dhev_recv(s.LH_q[idx] * tq[e], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rt[e], rl, Ft[e], Ed);
+                    if (e < ne) dhev_recv(s.ec, s.LH_q[idx] * tq[e], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rt[e], rl, Ft[e], Ed);
             });
             for_near<kCoreThreads>(st.ch_g, s.n_lhc * s.n_lh_g, tc, tr, ft, FT, s_list, s_wsum, [&](int x) {
                 const int m = x % s.n_lh_g;
                 const double3 Gx = ld3(st.G + 3 * x);
 #pragma unroll
                 for (int e = 0; e < kMaxRT; e++)
-                    if (e < ne) dhev_recv(s.LH_q[m] * tq[e], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], rt[e], Gx, Ft[e], Ed);
+                    if (e < ne) dhev_recv(s.ec, s.LH_q[m] * tq[e], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], rt[e], Gx, Ft[e], Ed);
             });
             double v[3 * kMaxRT];
 #pragma unroll
