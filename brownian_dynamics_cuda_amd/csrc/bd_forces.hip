@@ -1001,8 +1001,8 @@ __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, const dou
         const double *pk = Qp + 8 * ((size_t)k1 * Nq + k);
         const double3 Qk = make_double3(pk[0], pk[1], pk[2]);
         const double keq = s.k_e * pk[3];
-        // Fk = sum_l f_l and Gk = sum_l g_l (Q_l - r2), f_l = g_l (Qk - Q_l)
-        double3 Fk = zero3(), Gk = zero3();
+        // Fk = sum_l f_l, f_l = g_l (Qk - Q_l)
+        double3 Fk = zero3();
         double E = 0.0;
         const double *pl = Qp + 8 * (size_t)k2 * Nq;
         for (int l = l0; l < l1; l++) {
@@ -1015,7 +1015,7 @@ __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, const dou
             const double dist = r2s * inv;
             const double t = keq * d[3] * exp_neg_t(-kappa * dist, etab, ec);
             if (WITH_E) E += t * inv;
-            double g1d = t * (kappa * dist + 1.0) * (inv * inv * inv);
+            double g1d = t * (kappa + inv) * (inv * inv);  // (kappa d + 1) / d^3
             if (r2s <= kD8Sq) {  // the reference's d <= 8, exactly
                 const double x = sg * inv, x2 = x * x, x6 = x2 * x2 * x2, x7 = x6 * x, x13 = x6 * x7;
                 if (WITH_E) E += kex * (x6 * x6 - x6);
@@ -1024,19 +1024,15 @@ __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, const dou
             Fk.x = __builtin_fma(g1d, z.x, Fk.x);
             Fk.y = __builtin_fma(g1d, z.y, Fk.y);
             Fk.z = __builtin_fma(g1d, z.z, Fk.z);
-            Gk.x = __builtin_fma(g1d, d[4], Gk.x);
-            Gk.y = __builtin_fma(g1d, d[5], Gk.y);
-            Gk.z = __builtin_fma(g1d, d[6], Gk.z);
         }
         Fk = act * Fk;
-        Gk = act * Gk;
         const double3 a1 = ld3(g.a + 3 * b1), bb1 = ld3(g.b + 3 * b1), c1 = ld3(g.c + 3 * b1);
         const double3 T1 = torque_rel(Fk, ld3(s.cpos + 3 * k), a1, bb1, c1);
-        // torque on c2 about its centre from -f_l at lever (Q_l - r2):
-        // sum_l (Q_l - r2) x (-g_l (Qk - Q_l)) = -Gk x (Qk - r2), since
-        // (Q_l - r2) x (Q_l - r2) = 0
-        const double3 T2 = cross(Gk, Qk - ld3(g.r + 3 * s.core_bead[k2]));
-        double v[10] = {Fk.x, Fk.y, Fk.z, T1.x, T1.y, T1.z, -T2.x, -T2.y, -T2.z, act * E};
+        // torque on c2 about its centre r2 from -f_l at lever Q_l - r2 = w - z_l,
+        // w = Qk - r2, z_l = Qk - Q_l: sum_l (w - z_l) x (-g_l z_l) = Fk x w,
+        // since z_l x z_l = 0 -- the force sum alone gives it
+        const double3 T2 = cross(Fk, Qk - ld3(g.r + 3 * s.core_bead[k2]));
+        double v[10] = {Fk.x, Fk.y, Fk.z, T1.x, T1.y, T1.z, T2.x, T2.y, T2.z, act * E};
 #pragma unroll
         for (int q = 0; q < 10; q++)
             for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
