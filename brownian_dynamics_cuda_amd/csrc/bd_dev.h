@@ -83,8 +83,8 @@ __host__ __device__ inline int nchunks(int n) { return (n + kChunk - 1) / kChunk
 
 struct Sites {
     double *Q;  // 3 * n_c * Nq world positions of core charges
-    // 8 * n_c * Nq: per core charge {world position (3), charge,
-    // position - core centre (3), 0}, the records of the core-core sums
+    // 4 * n_c * Nq: per core charge {world position (3), charge}, the
+    // records of the core-core sums
     double *Qp;
     double *G;  // 3 * n_lhc * n_lh_g world positions of globular-head sites
     // bounding spheres (centre x, y, z, radius) of kChunk consecutive

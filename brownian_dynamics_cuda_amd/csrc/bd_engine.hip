@@ -475,7 +475,7 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     if (alloc_geo(c, c->cur) || alloc_geo(c, c->pred)) return bail(BD_E_HIP);
     if (alloc_frc(c, c->f0) || alloc_frc(c, c->f1) || alloc_frc(c, c->fa)) return bail(BD_E_HIP);
     r |= c->alloc(&c->sites.Q, 3 * (size_t)n_c * Nq);
-    r |= c->alloc(&c->sites.Qp, 8 * (size_t)n_c * Nq);
+    r |= c->alloc(&c->sites.Qp, 4 * (size_t)n_c * Nq);
     r |= c->alloc(&c->sites.G, 3 * (size_t)n_c * std::max(sys->n_lh_g, 1));
     r |= c->alloc(&c->sites.ch_t, nchunks(nt));
     r |= c->alloc(&c->sites.ch_lh, nchunks(c->lh_cap));

@@ -27,10 +27,8 @@ __global__ void k_sites(Stat s, Geo g, Sites st, int *cc_count) {
         double3 w = body_to_world(ld3(g.r + 3 * cb), ld3(g.a + 3 * cb), ld3(g.b + 3 * cb),
                                   ld3(g.c + 3 * cb), ld3(s.cpos + 3 * q));
         st3(st.Q + 3 * t, w);
-        const double3 rc = ld3(g.r + 3 * cb);
-        double *p = st.Qp + 8 * (size_t)t;
+        double *p = st.Qp + 4 * (size_t)t;
         p[0] = w.x; p[1] = w.y; p[2] = w.z; p[3] = s.cq[q];
-        p[4] = w.x - rc.x; p[5] = w.y - rc.y; p[6] = w.z - rc.z; p[7] = 0.0;
         return;
     }
     t -= nq;
@@ -998,15 +996,15 @@ __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, const dou
         const int k = min(kb * 64 + lane, Nq - 1);
         const double act = (kb * 64 + lane < Nq) ? 1.0 : 0.0;
         const int l0 = lc * chunk, l1 = min(Nq, l0 + chunk);
-        const double *pk = Qp + 8 * ((size_t)k1 * Nq + k);
+        const double *pk = Qp + 4 * ((size_t)k1 * Nq + k);
         const double3 Qk = make_double3(pk[0], pk[1], pk[2]);
         const double keq = s.k_e * pk[3];
         // Fk = sum_l f_l, f_l = g_l (Qk - Q_l)
         double3 Fk = zero3();
         double E = 0.0;
-        const double *pl = Qp + 8 * (size_t)k2 * Nq;
+        const double *pl = Qp + 4 * (size_t)k2 * Nq;
         for (int l = l0; l < l1; l++) {
-            const double *d = pl + 8 * l;
+            const double *d = pl + 4 * l;
             const double3 z = make_double3(Qk.x - d[0], Qk.y - d[1], Qk.z - d[2]);
             // Debye-Hueckel + excluded volume of the reference (dh_core, ev_core)
             // with 1/d from a refined reciprocal square root
