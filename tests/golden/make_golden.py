@@ -3,7 +3,8 @@
 Run in a container where the reference is present and oracle/_ref is built
 (``make -C oracle``):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py                  # every fixture
+    python tests/golden/make_golden.py negative_pivot   # that one only
 
 Every output here comes from the reference's own functions (func.cpp,
 mt.cpp, utilities.cpp) through oracle/_ref/libbdref.so; the inputs are
@@ -15,7 +16,14 @@ stored beside them so the fixtures stand alone:
   * tiny_deck.npz      -- a seeded 3-core fiber with tails and LH, same
     content, plus a second step;
   * overlap_tensor.npz -- beads closer than their radii (effective-radius
-    RPY branch) and a non-SPD tensor (the |f| pivot rule);
+    RPY branch), and eight strongly overlapping equal beads (still SPD:
+    the reference's tensor stayed positive definite in every overlapping
+    arrangement searched, with free radii too);
+  * negative_pivot.npz -- a 43-bead fiber (D 129 x 129, three 64-wide
+    tiles) with two beads given a negative radius, so D has negative
+    diagonal blocks and the factor takes the |f| branch of func.cpp:825-835
+    at six pivots, in the first and second tile; the reference's count of
+    those pivots is the number of lines it appends to wrongD.txt;
   * rng.npz            -- genrand_int32 / genrand_real1 / rand_normal
     sequences after init_genrand(5489).
 """
@@ -95,21 +103,46 @@ def deck_outputs(fib, st, nsteps=1):
 
 
 def overlap_case():
-    """Beads closer than the sum of their radii, and a tensor that is not
-    positive definite (pivot rule |f|)."""
+    """Beads closer than the sum of their radii, and eight strongly
+    overlapping equal beads (effective-radius branch, positive definite)."""
     fib, _ = Fb.make_fiber(3, seed=1)
     ref = O.Oracle(fib, "ref")
     r = np.array([0.0, 0.0, 0.0, 1.0, 0.5, 0.0, 2.2, 0.4, 0.3, 9.0, 0.0, 0.0])
     rad = np.array([1.5, 1.5, 5.0, 1.5])
     D, L, _ = ref.diffusion_tensor(r, rad)
-    # many strongly overlapping large beads: the RPY overlap branch is then
-    # not positive definite
+    # many strongly overlapping large beads (the overlap branch only)
     rng = np.random.default_rng(3)
     r2 = rng.uniform(0.0, 1.2, size=(8, 3)).reshape(-1)
     rad2 = np.full(8, 5.0)
     D2, L2, _ = ref.diffusion_tensor(r2, rad2)
     return dict(r=r, rad=rad, D=D, L=L, r2=r2, rad2=rad2, D2=D2, L2=L2,
                 a1=np.asarray(fib.phys.a1), a2=np.asarray(fib.phys.a2))
+
+
+NEG_BEADS = (5, 30)
+
+
+def negative_pivot_case():
+    """Diffusion_Tensor (func.cpp:728-836) of a 7-core fiber whose beads 5
+    and 30 carry a negative radius: their diagonal blocks a1/rad are
+    negative, every one of their six pivots is negative, and the factor
+    continues with |f|.  neg is the reference's own count (the lines of
+    wrongD.txt)."""
+    fib, st = Fb.make_fiber(7, seed=5)
+    rad = np.array(fib.rad, np.float64)
+    for b in NEG_BEADS:
+        rad[b] = -rad[b]
+    ref = O.Oracle(fib, "ref")
+    m = 3 * fib.n
+    D, L = np.zeros((m, m)), np.zeros((m, m))
+    f = fib.phys
+    with O.in_tempdir() as d:
+        ref.lib.ref_diffusion_tensor(fib.n, O._ptr(O._d(st.r)), O._ptr(rad), f.a1, f.a2,
+                                     O._ptr(D), O._ptr(L))
+        path = os.path.join(d, "wrongD.txt")
+        neg = sum(1 for _ in open(path)) if os.path.exists(path) else 0
+    return dict(r=np.asarray(st.r), rad=rad, D=D, L=L, neg=np.asarray(neg),
+                neg_beads=np.asarray(NEG_BEADS))
 
 
 def rng_case():
@@ -129,6 +162,9 @@ def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/libbdref.so is missing: build it with `make -C oracle` "
                  "where the reference is present")
+    if sys.argv[1:] == ["negative_pivot"]:
+        np.savez_compressed(os.path.join(HERE, "negative_pivot.npz"), **negative_pivot_case())
+        return
     if os.path.isdir(REF_DECK):
         fib, st = S.load_reference_deck(REF_DECK)
         d = pack_deck(fib, st)
@@ -140,6 +176,7 @@ def main():
     d.update(deck_outputs(fib, st, nsteps=2))
     np.savez_compressed(os.path.join(HERE, "tiny_deck.npz"), **d)
     np.savez_compressed(os.path.join(HERE, "overlap_tensor.npz"), **overlap_case())
+    np.savez_compressed(os.path.join(HERE, "negative_pivot.npz"), **negative_pivot_case())
     np.savez_compressed(os.path.join(HERE, "rng.npz"), **rng_case())
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

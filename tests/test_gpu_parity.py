@@ -308,3 +308,26 @@ def test_cfg3_forces_factor_and_long_chain():
     assert out[True][0] == out[False][0] and all(np.isfinite(out[True][0]))
     for k in ("r", "a", "b", "c", "r_t"):
         np.testing.assert_array_equal(getattr(out[True][1], k), getattr(out[False][1], k), err_msg=k)
+
+
+def test_negative_pivot_factor():
+    """The |f| rule (func.cpp:825-835) on the HIP factor: a 43-bead fiber
+    (D 129 x 129, three 64-wide tiles) whose beads 5 and 30 carry a negative
+    radius, so six pivots are negative -- three in the first tile, whose
+    flipped columns then feed the trailing updates of the other two, and
+    three in the second.  L and the count against the compiled reference
+    (tests/golden/negative_pivot.npz: its L and its wrongD.txt count)."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "negative_pivot.npz"))
+    fib, st = Fb.make_fiber(7, seed=5)
+    assert np.array_equal(st.r, z["r"])
+    fib.rad = z["rad"].copy()
+    for mode in (E.CHOL_FP64, E.CHOL_FP32):
+        with E.BDEngine(fib, chol_mode=mode, use_graph=False) as eng:
+            eng.set_state(st, rebuild=True)
+            D = eng.get_tensor()
+            eng.factor()
+            L, neg = eng.get_factor()
+        assert rel_err(D, z["D"]) <= 1e-14
+        assert neg == int(z["neg"]) == 6
+        assert rel_err(L, z["L"]) <= (TOL if mode == E.CHOL_FP64 else 1e-5)

@@ -1,0 +1,183 @@
+"""Closed-form checks of the oracle's step composition (or_step), which has
+no CPU function in the reference to compare with.  The step is rebuilt here
+in numpy/scipy from its pinned parts -- the RPY tensor and factor, the force
+terms, the frame update (all checked against the compiled reference in
+test_oracle_golden.py) -- and the glue of main_cuda (func_cuda.cu:3434-3680):
+
+  * rd = s2dt L p on the hydrodynamic rows, s2dt sqrt(a1/rad) p on the
+    others (rd_cal, func_cuda.cu:634-647);
+  * x' = x0 + rd + del D_blk F with D_blk the bead's own species block
+    (translation_cal, func_cuda.cu:650-679);
+  * dtheta = dt tau / zeta + rr, zeta_core = 8 pi eta 125 and
+    zeta_linker = 4 pi eta r_h^2 lo (constants.h:6-12), linker beads keeping
+    only the a (twist) component (rotation_cal, func_cuda.cu:681-700), the
+    body frame turned by the rotation vector dtheta (Rodrigues, scipy);
+  * predictor from (F0, tau0), force pass 2 at the predicted state, the
+    corrector from x0 with 1/2 (F0 + F1), 1/2 (tau0 + tau1) and the same rd
+    and rr (func_cuda.cu:3617-3654);
+  * energy of pass 1 only (func_cuda.cu:3613-3615)."""
+import math
+
+import numpy as np
+import pytest
+from scipy.spatial.transform import Rotation
+
+import oracle_py as O
+from brownian_dynamics_cuda_amd import system as S
+from decks import deck, rel_err
+
+# constants.h:6-12
+PI, ETA, LO, R_H = 3.141592653589790, 1.137076e-12, 3.0, 1.2
+ZETA_CORE = 8 * PI * ETA * 125.0
+ZETA_LINKER = 4 * PI * ETA * R_H * R_H * LO
+
+
+def species_blocks(fib):
+    """(offset, count) of the main, tail and LH blocks of the all-bead vector."""
+    n3, t3, l3 = 3 * fib.n, 3 * fib.n_tail, 3 * fib.n_LH
+    return [(0, n3), (n3, t3), (n3 + t3, l3)]
+
+
+def drift(fib, D, F_all, rad_all):
+    """del D_blk F: the bead's own species block of D where the bead is in
+    the hydrodynamic set, its self-mobility a1/rad otherwise."""
+    m = D.shape[0]
+    out = np.zeros(F_all.size)
+    dl, a1 = fib.phys.dell, fib.phys.a1
+    for off, cnt in species_blocks(fib):
+        if cnt == 0:
+            continue
+        sl = slice(off, off + cnt)
+        if off < m:
+            out[sl] = dl * (D[sl, sl] @ F_all[sl])
+        else:
+            out[sl] = dl * (a1 / np.repeat(rad_all[off // 3:(off + cnt) // 3], 3)) * F_all[sl]
+    return out
+
+
+def rotate_frames(fib, a, b, c, T, rr):
+    """Frames turned by dtheta = dt T / zeta + rr in body coordinates."""
+    dt = fib.phys.time_step
+    a2, b2, c2 = a.copy(), b.copy(), c.copy()
+    for i in range(fib.n):
+        s = slice(3 * i, 3 * i + 3)
+        if fib.types[i] != 0:
+            w = dt * T[s] / ZETA_CORE + rr[s]
+        else:
+            w = np.array([dt * T[3 * i] / ZETA_LINKER + rr[3 * i], 0.0, 0.0])
+        Fm = np.stack([a[s], b[s], c[s]], 1)
+        G = Fm @ Rotation.from_rotvec(w).as_matrix()
+        a2[s], b2[s], c2[s] = G[:, 0], G[:, 1], G[:, 2]
+    return a2, b2, c2
+
+
+def composed_state(fib, fs0, rd, D, F, T, tF, lF, rr):
+    """Translate and rotate from fs0 with the given forces, then rebuild the
+    derived arrays with the oracle's (pinned) frame and bending updates."""
+    rad_all = fib.rad_all()
+    F_all = np.concatenate([F, tF, lF])
+    x_all = np.concatenate([fs0.view("r"), fs0.view("r_t"), fs0.view("r_lh")])
+    x_new = x_all + rd + drift(fib, D, F_all, rad_all)
+    a, b, c = rotate_frames(fib, fs0.view("a"), fs0.view("b"), fs0.view("c"), T, rr)
+    n3, t3 = 3 * fib.n, 3 * fib.n_tail
+    st = S.State(x_new[:n3], a, b, c, x_new[n3:n3 + t3], x_new[n3 + t3:])
+    orc = O.Oracle(fib)
+    return orc.prepare(st)
+
+
+CASES = [("tiny", 0), ("tiny", 1), ("small", 0), ("small", 1)]
+
+
+@pytest.mark.parametrize("kind,hi", CASES)
+def test_noise_displacement_closed_form(kind, hi):
+    fib, st = deck(kind, hi_all=hi)
+    orc = O.Oracle(fib)
+    fs = orc.prepare(st)
+    rr, p = O.Rng(5489).step_noise(0, fib.r_var(), 3 * (fib.n + fib.n_tail + fib.n_LH))
+    out = orc.step(fs.copy(), rr, p, want=True)
+    m = 3 * fib.n_hi
+    L = out["L"].reshape(m, m)
+    ref = np.empty_like(p)
+    ref[:m] = fib.phys.s2dt * (L @ p[:m])
+    rad_all = fib.rad_all()
+    ref[m:] = fib.phys.s2dt * np.sqrt(fib.phys.a1 / np.repeat(rad_all, 3)[m:]) * p[m:]
+    assert rel_err(out["rd"], ref) <= 1e-14
+    # and L is the LAPACK factor of D
+    D = out["D"].reshape(m, m)
+    assert rel_err(L, np.linalg.cholesky(D)) <= 1e-12
+
+
+@pytest.mark.parametrize("kind,hi", CASES)
+def test_predictor_corrector_composition(kind, hi):
+    """The oracle's step against the composition rebuilt here: force pass 2
+    at the predicted state, and the committed state from the Heun average."""
+    fib, st = deck(kind, hi_all=hi)
+    orc = O.Oracle(fib)
+    fs0 = orc.prepare(st)
+    rr, p = O.Rng(777).step_noise(0, fib.r_var(), 3 * (fib.n + fib.n_tail + fib.n_LH))
+    fs = fs0.copy()
+    out = orc.step(fs, rr, p, want=True)
+    m = 3 * fib.n_hi
+    D, rd = out["D"].reshape(m, m), out["rd"]
+    nt3, nl3 = 3 * fib.n_tail, 3 * fib.n_LH
+    F0, T0, tF0, lF0 = out["F0"], out["T0"], out["tF0"][:nt3], out["lhF0"][:nl3]
+    F1, T1, tF1, lF1 = out["F1"], out["T1"], out["tF1"][:nt3], out["lhF1"][:nl3]
+
+    # pass 1 is the force pass at x0, and its energy is the step's energy
+    f0 = orc.forces(fs0)
+    for got, key in ((F0, "F"), (T0, "T"), (tF0, "tF"), (lF0, "lhF")):
+        assert rel_err(got, f0[key]) <= 1e-15, key
+    assert out["E"] == f0["E"]
+
+    # predictor: x0 + rd + del D_blk F0, frames by dt T0 / zeta + rr; pass 2 there
+    pred = composed_state(fib, fs0, rd, D, F0, T0, tF0, lF0, rr)
+    f1 = orc.forces(pred)
+    for got, key in ((F1, "F"), (T1, "T"), (tF1, "tF"), (lF1, "lhF")):
+        assert rel_err(got, f1[key]) <= 1e-9, key
+
+    # corrector: from x0 with the averaged forces and torques, same rd, rr
+    avg = lambda u, v: 0.5 * (u + v)
+    fin = composed_state(fib, fs0, rd, D, avg(F0, F1), avg(T0, T1), avg(tF0, tF1), avg(lF0, lF1),
+                         rr)
+    for k in ("r", "r_t", "r_lh"):
+        assert rel_err(fs.view(k), fin.view(k)) <= 1e-14, k
+    for k in ("a", "b", "c", "length", "a_dna", "b_dna", "c_dna", "beta_t", "beta_lh"):
+        assert rel_err(fs.view(k), fin.view(k)) <= 1e-12, k
+
+
+def test_rotation_zeta_and_linker_component():
+    """The frame increment alone: zero forces and torques except one torque
+    per bead, no noise; a core turns by dt tau / zeta_core about all three
+    body axes, a linker bead by dt tau_a / zeta_linker about its a axis only."""
+    fib, st = deck("tiny")
+    orc = O.Oracle(fib)
+    fs0 = orc.prepare(st)
+    rng = np.random.default_rng(4)
+    T = rng.normal(size=3 * fib.n) * 1e-3
+    rr = np.zeros(3 * fib.n)
+    a, b, c = rotate_frames(fib, fs0.view("a"), fs0.view("b"), fs0.view("c"), T, rr)
+    lib = O.oracle_lib()
+    # the oracle's rotate with the increment formed by hand from constants.h
+    dth = np.zeros(3 * fib.n)
+    for i in range(fib.n):
+        if fib.types[i] != 0:
+            dth[3 * i:3 * i + 3] = fib.phys.time_step * T[3 * i:3 * i + 3] / ZETA_CORE
+        else:
+            dth[3 * i] = fib.phys.time_step * T[3 * i] / ZETA_LINKER
+    outs = [np.zeros(3 * fib.n) for _ in range(3)]
+    p = O._ptr
+    lib.or_rotate(fib.n, p(O._d(fs0.view("a"))), p(O._d(fs0.view("b"))), p(O._d(fs0.view("c"))),
+                  p(outs[0]), p(outs[1]), p(outs[2]), p(dth), 1.0)
+    for got, ref in zip(outs, (a, b, c)):
+        assert rel_err(got, ref) <= 1e-14
+    # the rotational variance r_var = 2 dt kT / zeta (main.cpp:288-298)
+    rv = fib.r_var()
+    kT = fib.phys.kbt
+    for i in range(fib.n):
+        if fib.types[i] != 0:
+            assert rv[3 * i:3 * i + 3] == pytest.approx([2 * fib.phys.time_step * kT / ZETA_CORE] * 3,
+                                                        rel=1e-15)
+        else:
+            assert rv[3 * i] == pytest.approx(2 * fib.phys.time_step * kT / ZETA_LINKER, rel=1e-15)
+            assert rv[3 * i + 1] == 0 and rv[3 * i + 2] == 0
+    assert math.isclose(ZETA_CORE / ZETA_LINKER, 2 * 125.0 / (R_H * R_H * LO))

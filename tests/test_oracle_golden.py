@@ -75,6 +75,36 @@ def test_overlap_tensor():
     D2, L2, neg2 = orc.diffusion_tensor(z["r2"], z["rad2"])
     assert rel_err(D2, z["D2"]) <= 1e-15
     assert rel_err(L2, z["L2"]) <= 1e-12
+    # eight overlapping equal beads: positive definite, no |f| pivot
+    assert neg == 0 and neg2 == 0
+    assert np.linalg.eigvalsh(D2).min() > 0
+
+
+def test_negative_pivot_rule():
+    """The |f| branch of the reference's Cholesky (func.cpp:825-835) on a
+    129 x 129 tensor with six negative pivots in two tiles (two beads with a
+    negative radius): the oracle's L and count against the reference's, and
+    the closed form of the rule -- the factor is the exact Cholesky factor of
+    D + 2 diag(|f_i|) over the flipped pivots i, so L L^T - D vanishes off
+    the diagonal and equals 2 L_ii^2 on the flipped rows."""
+    z = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
+                                           "negative_pivot.npz"))
+    from brownian_dynamics_cuda_amd import fiber as Fb
+    fib, _ = Fb.make_fiber(7, seed=5)
+    orc = O.Oracle(fib, "or")
+    D, L, neg = orc.diffusion_tensor(z["r"], z["rad"])
+    assert rel_err(D, z["D"]) <= 1e-15
+    assert rel_err(L, z["L"]) <= 1e-13
+    assert neg == int(z["neg"]) == 6
+    flipped = np.concatenate([np.arange(3 * b, 3 * b + 3) for b in z["neg_beads"]])
+    R = L @ L.T - D
+    off = R - np.diag(np.diag(R))
+    assert np.max(np.abs(off)) <= 1e-13 * np.max(np.abs(D))
+    expect = np.zeros(D.shape[0])
+    expect[flipped] = 2 * np.diag(L)[flipped] ** 2
+    assert rel_err(np.diag(R), expect) <= 1e-12
+    # the flipped pivots are exactly the negative diagonal blocks' rows
+    assert np.all(np.diag(D)[flipped] < 0) and np.sum(np.diag(D) < 0) == 6
 
 
 def test_rng_matches_reference():
