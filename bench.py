@@ -173,34 +173,55 @@ def main():
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = read_traffic(args.traffic_file, dominant)
 
-    # mixed-precision factor: same state and noise, one step each way
+    # mixed-precision factors: same state and noise, one step each way;
+    # displacement error against FP64, factor error ||L L^T - D|| / ||D||
+    # (through random probes L (L^T v) - D v above 4096 rows), steps/s
     mixed = None
     if not args.no_mixed:
+        mixed = {}
         eng64 = E.BDEngine(fib, chol_mode=E.CHOL_FP64, use_graph=False)
-        eng32 = E.BDEngine(fib, chol_mode=E.CHOL_FP32, use_graph=True)
-        for e_ in (eng64, eng32):
-            e_.set_state(st, rebuild=True)
+        eng64.set_state(st, rebuild=True)
         x0 = eng64.get_state()
         eng64.step(rr[0], p[0])
-        eng32.step(rr[0], p[0])
-        a, b = eng64.get_state(), eng32.get_state()
+        a = eng64.get_state()
         disp64 = np.concatenate([a.r - x0.r, a.r_t - x0.r_t, a.r_lh - x0.r_lh])
-        disp32 = np.concatenate([b.r - x0.r, b.r_t - x0.r_t, b.r_lh - x0.r_lh])
-        rel = float(np.linalg.norm(disp32 - disp64) / np.linalg.norm(disp64))
-        eng32.set_state(st, rebuild=True)
-        d_e2 = E.DeviceBuffer(8 * (W + K))
-        for s in range(W):
-            eng32.step_device(d_rr.ptr + 8 * n3 * s, d_p.ptr + 8 * nD3 * s, d_e2.ptr + 8 * s)
-        eng32.synchronize()
-        t1 = time.perf_counter()
-        for s in range(W, W + K):
-            eng32.step_device(d_rr.ptr + 8 * n3 * s, d_p.ptr + 8 * nD3 * s, d_e2.ptr + 8 * s)
-        eng32.synchronize()
-        d32 = time.perf_counter() - t1
-        mixed = dict(chol="fp32-trailing", steps_per_s=K / d32, ms_per_step=1e3 * d32 / K,
-                     rel_displacement_err_vs_fp64=rel)
         eng64.close()
-        eng32.close()
+        for mode, name in ((E.CHOL_FP32, "fp32"), (E.CHOL_BF16, "bf16x3")):
+            engm = E.BDEngine(fib, chol_mode=mode, use_graph=True)
+            engm.set_state(st, rebuild=True)
+            engm.step(rr[0], p[0])
+            b = engm.get_state()
+            disp = np.concatenate([b.r - x0.r, b.r_t - x0.r_t, b.r_lh - x0.r_lh])
+            rel = float(np.linalg.norm(disp - disp64) / np.linalg.norm(disp64))
+            engm.set_state(st, rebuild=True)
+            D = engm.get_tensor()
+            engm.factor()
+            L, _ = engm.get_factor()
+            if D.shape[0] <= 4096:
+                ferr = float(np.linalg.norm(L @ L.T - D) / np.linalg.norm(D))
+            else:
+                g = np.random.default_rng(0)
+                ferr = 0.0
+                for _ in range(3):
+                    v = g.standard_normal(D.shape[0])
+                    Dv = D @ v
+                    ferr = max(ferr, float(np.linalg.norm(L @ (L.T @ v) - Dv) / np.linalg.norm(Dv)))
+            del D, L
+            engm.set_state(st, rebuild=True)
+            d_e2 = E.DeviceBuffer(8 * (W + K))
+            for s in range(W):
+                engm.step_device(d_rr.ptr + 8 * n3 * s, d_p.ptr + 8 * nD3 * s, d_e2.ptr + 8 * s)
+            engm.synchronize()
+            t1 = time.perf_counter()
+            for s in range(W, W + K):
+                engm.step_device(d_rr.ptr + 8 * n3 * s, d_p.ptr + 8 * nD3 * s, d_e2.ptr + 8 * s)
+            engm.synchronize()
+            dm = time.perf_counter() - t1
+            mixed[name] = dict(steps_per_s=K / dm, ms_per_step=1e3 * dm / K,
+                               rel_displacement_err_vs_fp64=rel, factor_rel_err=ferr)
+            engm.close()
+            print(f"[bench] {name}: {K / dm:.2f} steps/s, displacement err {rel:.2e}, "
+                  f"factor err {ferr:.2e}", file=sys.stderr, flush=True)
 
     # CPU baseline: the reference's own func.cpp composed into one step
     cpu = None

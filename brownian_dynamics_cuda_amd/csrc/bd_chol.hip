@@ -10,11 +10,18 @@
 // a bounds test).  Right-looking blocked algorithm with 64x64 tiles, one
 // launch per block column kb (k_step): every workgroup forms the two panel
 // tiles it needs as A_ib,kb inv(L_kb,kb)^T (FP64 matrix cores), updates its
-// trailing tile A_ib,jb -= L_ib,kb L_jb,kb^T (FP64 or FP32 matrix cores),
-// and the workgroup owning tile (kb+1,kb+1) then factors it and inverts
-// the factor (four wavefronts, FP64, the inverse built beside the pivots).
-// The solved off-diagonal tile L_ib,kb stays parked in the upper block
-// (kb, ib) of the same storage.
+// trailing tile A_ib,jb -= L_ib,kb L_jb,kb^T (FP64, FP32 or three-slice
+// BF16 matrix-core products), and the workgroup owning tile (kb+1,kb+1)
+// then factors it and inverts the factor (four wavefronts, FP64, the
+// inverse built beside the pivots).  The solved off-diagonal tile L_ib,kb
+// stays parked in the upper block (kb, ib) of the same storage.
+//
+// Large matrices take super-panels of NB block columns instead: the panel
+// columns are factored as above with their updates restricted to the panel,
+// and the trailing matrix then receives the whole panel in one launch
+// (k_trail: each tile accumulates K = 64 NB in registers before its one
+// read-modify-write), so the trailing matrix streams through HBM once per
+// NB block columns rather than once per block column.
 #include "bd_dev.h"
 #include "bd_kernels.h"
 
@@ -25,6 +32,7 @@ constexpr int LDP = TB + 1;   // LDS row pitch (doubles)
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 static inline int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
 
@@ -207,6 +215,65 @@ __device__ __forceinline__ void mma_f32(const double *As, const double *Bs, flt4
         acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
     }
 }
+
+// Three-slice BF16 form of the same product.  Each operand x is rounded to
+// FP32 and split exactly into three BF16 numbers, x_f32 = h + m + l
+// (h = bf16(x), m = bf16(x - h), l = x - h - m, which has at most 8
+// significant bits and is exact); the six products whose weight reaches
+// 2^-24 -- hh, hm, mh, hl, mm, lh -- are accumulated in FP32 on
+// v_mfma_f32_16x16x32_bf16, smallest first.  The result carries FP32-class
+// accuracy from BF16 matrix-core products.  Lane l supplies row l%16 and
+// the eight K entries 8(l/16)..8(l/16)+7 of a 32-wide K step (the same K
+// mapping for both operands, so the sum is the same whatever the hardware's
+// K order); results as in the FP32 form (row 4(l/16) + v).
+__device__ __forceinline__ void split3(const double *src, bf8 &h, bf8 &m, bf8 &l) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const float f = (float)src[e];
+        const __bf16 hh = (__bf16)f;
+        const float r1 = f - (float)hh;
+        const __bf16 mm = (__bf16)r1;
+        h[e] = hh;
+        m[e] = mm;
+        l[e] = (__bf16)(r1 - (float)mm);
+    }
+}
+__device__ __forceinline__ void mma_bf16x3(const double *As, const double *Bs, flt4 acc[2][2]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32, i16 = lane & 15, k8 = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < TB; kk += 32) {
+        bf8 ah[2], am[2], al[2], bh[2], bm[2], bl[2];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            split3(As + (wr + 16 * q + i16) * LDP + kk + 8 * k8, ah[q], am[q], al[q]);
+            split3(Bs + (wc + 16 * q + i16) * LDP + kk + 8 * k8, bh[q], bm[q], bl[q]);
+        }
+#pragma unroll
+        for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+            for (int ni = 0; ni < 2; ni++) {
+                flt4 c = acc[mi][ni];
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], bh[ni], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[mi], bm[ni], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bl[ni], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[mi], bh[ni], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bm[ni], c, 0, 0, 0);
+                acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bh[ni], c, 0, 0, 0);
+            }
+    }
+}
+
+// acc (-)= As Bs^T for one 64x64 tile in the arithmetic of MODE
+// (BD_CHOL_FP64 / FP32 / BF16); Acc is dbl4 for FP64, flt4 otherwise.
+template <int MODE, typename Acc>
+__device__ __forceinline__ void tile_mma(const double *As, const double *Bs, Acc acc[2][2]) {
+    if constexpr (MODE == 0) mma_f64(As, Bs, acc);
+    else if constexpr (MODE == 1) mma_f32(As, Bs, acc);
+    else mma_bf16x3(As, Bs, acc);
+}
+template <int MODE> struct AccOf { typedef flt4 T; };
+template <> struct AccOf<0> { typedef dbl4 T; };
 
 // C (LDS) += sign * acc, element mapping of mma_* (row of register v of
 // lane l: l/16 + 4v for the FP64 MFMA, 4(l/16) + v for the FP32 one)
@@ -713,23 +780,38 @@ __device__ __forceinline__ void diag_factor_inv(double *T, double *LP, double *X
 }
 
 
+// Trailing tiles of block column kb restricted to the first jcols trailing
+// block columns, row-major: rows ii = 0..M-1, columns jj <= min(ii, jcols-1)
+// (jcols = M: the whole trailing triangle, in tri_index order).
+__device__ __forceinline__ void trap_index(int idx, int jcols, int &i, int &j) {
+    const int tri = jcols * (jcols + 1) / 2;
+    if (idx < tri) {
+        tri_index(idx, i, j);
+        return;
+    }
+    idx -= tri;
+    i = jcols + idx / jcols;
+    j = idx % jcols;
+}
+
 // One block column kb of the factorisation in one launch, one workgroup per
-// trailing lower tile (ib, jb), kb < jb <= ib:
+// trailing lower tile (ib, jb), kb < jb <= ib, jb - kb <= jcols:
 //   1. the panel tiles L_ib,kb = A_ib,kb inv(L_kb,kb)^T and L_jb,kb on the
 //      FP64 matrix cores, from the inverse the previous launch left in
-//      Linv; the workgroup of the diagonal tile (ib, ib) stores L_ib,kb, so
-//      every panel tile is written exactly once, into the upper block
+//      Linv; the workgroup of the first column (jb = kb + 1) stores L_ib,kb,
+//      so every panel tile is written exactly once, into the upper block
 //      (kb, ib), because other workgroups of this launch still read
-//      A_ib,kb; the tiles stay there (k_noise and the host copy of the
-//      factor read them from there);
-//   2. A_ib,jb -= L_ib,kb L_jb,kb^T on the matrix cores (FP64, or FP32 with
-//      FP32 accumulation when MODE != 0);
+//      A_ib,kb; the tiles stay there (k_noise, k_trail and the host copy
+//      of the factor read them from there);
+//   2. A_ib,jb -= L_ib,kb L_jb,kb^T on the matrix cores (FP64, or FP32 /
+//      three-slice BF16 with FP32 accumulation for MODE 1 / 2);
 //   3. the workgroup of tile (kb+1, kb+1) factors and inverts it, which the
 //      next launch needs.
 // The panel products are repeated by the workgroups of a block row, in
 // exchange for one launch (and one kernel boundary) per block column.
+// jcols < M restricts the update to the columns of a super-panel.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, double *Linv, int *neg) {
+__global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, int jcols, double *Linv, int *neg) {
     __shared__ double W[TB * LDP], As[TB * LDP], Bs[TB * LDP], Cs[TB * LDP];
     // workgroups go round-robin over the eight XCDs (b % 8); XCD x takes a
     // contiguous run of tiles, so the tiles of a block row, which read the
@@ -738,7 +820,7 @@ __global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, double 
     const int G8 = gridDim.x / 8;
     if (lb < 8 * G8) lb = (lb % 8) * G8 + lb / 8;
     int ii, jj;
-    tri_index(lb, ii, jj);
+    trap_index(lb, jcols, ii, jj);
     const int ib = kb + 1 + ii, jb = kb + 1 + jj;
     const bool diag = ib == jb;
     {
@@ -760,22 +842,14 @@ __global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, double 
         if (!diag) panel_to_lds(Bs, pj);
     }
     __syncthreads();
-    if (diag) store_tile(A, ld, kb * TB, ib * TB, As, false);
-    if (MODE == 0) {
-        if (diag) {
-            syrk_lower(As, Cs);
-        } else {
-            dbl4 acc[2][2];
-            for (int a = 0; a < 2; a++)
-                for (int b = 0; b < 2; b++) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-            mma_f64(As, Bs, acc);
-            acc_to_lds(Cs, acc, -1.0, false);
-        }
+    if (jj == 0) store_tile(A, ld, kb * TB, ib * TB, As, false);
+    if (MODE == 0 && diag) {
+        syrk_lower(As, Cs);
     } else {
-        flt4 acc[2][2];
+        typename AccOf<MODE>::T acc[2][2];
         for (int a = 0; a < 2; a++)
-            for (int b = 0; b < 2; b++) acc[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
-        mma_f32(As, diag ? As : Bs, acc);
+            for (int b = 0; b < 2; b++) acc[a][b] = {};
+        tile_mma<MODE>(As, diag ? As : Bs, acc);
         acc_to_lds(Cs, acc, -1.0, false);
     }
     __syncthreads();
@@ -831,21 +905,13 @@ __global__ void __launch_bounds__(256) k_update(double *A, int ld, int kb, doubl
         load_tiles(diag ? 2 : 3, src, lds, dst);
     }
     __syncthreads();
-    if (MODE == 0) {
-        if (diag) {
-            syrk_lower(As, Cs);
-        } else {
-            dbl4 acc[2][2];
-            for (int a = 0; a < 2; a++)
-                for (int b = 0; b < 2; b++) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-            mma_f64(As, Bs, acc);
-            acc_to_lds(Cs, acc, -1.0, false);
-        }
+    if (MODE == 0 && diag) {
+        syrk_lower(As, Cs);
     } else {
-        flt4 acc[2][2];
+        typename AccOf<MODE>::T acc[2][2];
         for (int a = 0; a < 2; a++)
-            for (int b = 0; b < 2; b++) acc[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
-        mma_f32(As, diag ? As : Bs, acc);
+            for (int b = 0; b < 2; b++) acc[a][b] = {};
+        tile_mma<MODE>(As, diag ? As : Bs, acc);
         acc_to_lds(Cs, acc, -1.0, false);
     }
     __syncthreads();
@@ -853,6 +919,80 @@ __global__ void __launch_bounds__(256) k_update(double *A, int ld, int kb, doubl
         diag_factor_inv(Cs, As, Bs, A, ld, ib * TB, Linv + (size_t)ib * TB * TB, neg);
     else
         store_tile(A, ld, ib * TB, jb * TB, Cs, diag);
+}
+
+// Trailing update of a super-panel: block columns k0 .. k0+nb-1 are
+// factored and their panel tiles parked; every lower tile (ib, jb) of the
+// trailing matrix (rows and columns from j0 = k0 + nb) takes
+//   A_ib,jb -= sum_k L_ib,k L_jb,k^T   (K = 64 nb)
+// with the sum held in registers across the nb panel tiles and the tile
+// read and written once.  The next panel tiles are loaded into registers
+// while the current ones are multiplied (the target tile during the last).
+// Workgroups are grouped in 8x8 blocks of tiles, 64 consecutive workgroups
+// per block, so the panel tiles of a block's 8 rows and 8 columns are
+// shared through the L2; workgroups past the triangle or the edge exit.
+constexpr int NTHR_T = 256;
+__device__ __forceinline__ void tile_to_regs(const double *src, int ld, dbl2 (&v)[TB * TB / 2 / NTHR_T]) {
+#pragma unroll
+    for (int i = 0; i < TB * TB / 2 / NTHR_T; i++) {
+        const int e = threadIdx.x + i * NTHR_T, r = e / (TB / 2), c = 2 * (e % (TB / 2));
+        v[i] = *reinterpret_cast<const dbl2 *>(src + (size_t)r * ld + c);
+    }
+}
+__device__ __forceinline__ void regs_to_lds(const dbl2 (&v)[TB * TB / 2 / NTHR_T], double *S) {
+#pragma unroll
+    for (int i = 0; i < TB * TB / 2 / NTHR_T; i++) {
+        const int e = threadIdx.x + i * NTHR_T, r = e / (TB / 2), c = 2 * (e % (TB / 2));
+        S[r * LDP + c] = v[i].x;
+        S[r * LDP + c + 1] = v[i].y;
+    }
+}
+template <int MODE>
+__global__ void __launch_bounds__(256, 2) k_trail(double *A, int ld, int k0, int nb, int j0, int nt) {
+    __shared__ double As[TB * LDP], Bs[TB * LDP];
+    int SI, SJ;
+    tri_index(blockIdx.x >> 6, SI, SJ);
+    const int ib = j0 + 8 * SI + ((blockIdx.x >> 3) & 7), jb = j0 + 8 * SJ + (blockIdx.x & 7);
+    if (ib >= nt || jb > ib) return;
+    const bool diag = ib == jb;
+    constexpr int PER = TB * TB / 2 / NTHR_T;
+    dbl2 va[PER], vb[PER];
+    typename AccOf<MODE>::T acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = {};
+    // L_ib,k is parked at block (k, ib)
+    tile_to_regs(A + (size_t)k0 * TB * ld + ib * TB, ld, va);
+    if (!diag) tile_to_regs(A + (size_t)k0 * TB * ld + jb * TB, ld, vb);
+    for (int t = 0; t < nb; t++) {
+        regs_to_lds(va, As);
+        if (!diag) regs_to_lds(vb, Bs);
+        __syncthreads();
+        if (t + 1 < nb) {
+            const size_t row = (size_t)(k0 + t + 1) * TB * ld;
+            tile_to_regs(A + row + ib * TB, ld, va);
+            if (!diag) tile_to_regs(A + row + jb * TB, ld, vb);
+        } else {
+            tile_to_regs(A + (size_t)ib * TB * ld + jb * TB, ld, va);
+        }
+        tile_mma<MODE>(As, diag ? As : Bs, acc);
+        __syncthreads();
+    }
+    regs_to_lds(va, As);
+    __syncthreads();
+    acc_to_lds(As, acc, -1.0, false);
+    __syncthreads();
+    store_tile(A, ld, ib * TB, jb * TB, As, diag);
+}
+
+// Factor and inverse of the diagonal tile (d, d) by one workgroup (the
+// tile after a super-panel's trailing update, or the first tile of a
+// caller's matrix); clear != 0 also restarts the negative-pivot count.
+__global__ void __launch_bounds__(256) k_diag(double *A, int ld, int d, double *Linv, int *neg, int clear) {
+    __shared__ double T[TB * LDP], LP[TB * TB], X[TB * LDP];
+    if (clear && threadIdx.x == 0) *neg = 0;
+    load_tile(A, ld, d * TB, d * TB, T);
+    __syncthreads();
+    diag_factor_inv(T, LP, X, A, ld, d * TB, Linv + (size_t)d * TB * TB, neg);
 }
 
 // rd = s2dt * L * p (rd_cal, func_cuda.cu:634-647): one wavefront per row,
@@ -903,27 +1043,69 @@ void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, co
     (void)n_hi;
 }
 
+namespace {
+template <int MODE>
+void chol_column(const CholWork &c, int kb, int jcols, hipStream_t q) {
+    const int nt = c.mpad / TB, M = nt - kb - 1;
+    if (jcols == 0) {
+        k_panel<<<M, 256, 0, q>>>(c.A, c.ld, kb, c.Linv);
+    } else if (jcols == M && M * (M + 1) / 2 >= c.split_tiles) {
+        k_panel<<<M, 256, 0, q>>>(c.A, c.ld, kb, c.Linv);
+        k_update<MODE><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
+    } else {
+        const int grid = jcols * (jcols + 1) / 2 + (M - jcols) * jcols;
+        k_step<MODE><<<grid, 256, 0, q>>>(c.A, c.ld, kb, jcols, c.Linv, c.neg);
+    }
+}
+template <int MODE>
+void chol_trail(const CholWork &c, int k0, int nb, hipStream_t q) {
+    const int nt = c.mpad / TB, j0 = k0 + nb, SM = (nt - j0 + 7) / 8;
+    k_trail<MODE><<<SM * (SM + 1) / 2 * 64, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, nt);
+}
+}  // namespace
+
+int chol_panel_tiles(const CholWork &c) {
+    const int nt = c.mpad / TB;
+    if (c.panel_tiles > 0) return c.panel_tiles >= 2 && nt > c.panel_tiles ? c.panel_tiles : 0;
+    if (c.panel_tiles < 0) return 0;
+    return nt >= 64 ? 8 : 0;
+}
+
 void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_arg, hipEvent_t gate, int gate_kb) {
     const int nt = c.mpad / TB;
     auto M_ = [&](int id, int ph) { if (mark) mark(mark_arg, id, ph); };
-    for (int kb = 0; kb + 1 < nt; kb++) {
-        const int M = nt - kb - 1;
+    auto column = [&](int kb, int jcols) {
         M_(K_CHOL_STEP, 0);
-        if (M * (M + 1) / 2 >= c.split_tiles) {
-            k_panel<<<M, 256, 0, q>>>(c.A, c.ld, kb, c.Linv);
-            if (c.mode == 0)
-                k_update<0><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
-            else
-                k_update<1><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
-        } else if (c.mode == 0) {
-            k_step<0><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
-        } else {
-            k_step<1><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
-        }
+        if (c.mode == 0) chol_column<0>(c, kb, jcols, q);
+        else if (c.mode == 1) chol_column<1>(c, kb, jcols, q);
+        else chol_column<2>(c, kb, jcols, q);
         M_(K_CHOL_STEP, 1);
         if (gate && kb == gate_kb) (void)hipEventRecord(gate, q);
+    };
+    const int NB = chol_panel_tiles(c);
+    int k0 = 0;
+    if (NB > 0) {
+        // super-panels while a whole one and a trailing matrix remain
+        for (; k0 + NB < nt; k0 += NB) {
+            const int kend = k0 + NB;
+            for (int kb = k0; kb < kend; kb++) column(kb, kend - 1 - kb);
+            M_(K_CHOL_TRAIL, 0);
+            if (c.mode == 0) chol_trail<0>(c, k0, NB, q);
+            else if (c.mode == 1) chol_trail<1>(c, k0, NB, q);
+            else chol_trail<2>(c, k0, NB, q);
+            M_(K_CHOL_TRAIL, 1);
+            M_(K_CHOL_DIAG, 0);
+            k_diag<<<1, 256, 0, q>>>(c.A, c.ld, kend, c.Linv, c.neg, 0);
+            M_(K_CHOL_DIAG, 1);
+        }
     }
+    // one launch (or two) per block column, each updating the whole trailing matrix
+    for (int kb = k0; kb + 1 < nt; kb++) column(kb, nt - kb - 1);
     if (gate && gate_kb >= nt - 1) (void)hipEventRecord(gate, q);
+}
+
+void launch_diag(const CholWork &c, int d, bool clear, hipStream_t q) {
+    k_diag<<<1, 256, 0, q>>>(c.A, c.ld, d, c.Linv, c.neg, clear ? 1 : 0);
 }
 
 void launch_noise(const Stat &s, int n_hi, int n_D3, const CholWork &c, const double *p, const double *rad_all,

@@ -56,7 +56,7 @@ int fail(int code, const char *fmt, ...) {
 const char *kKernelNames[K_COUNT] = {
     "sites", "mech", "linker", "tail", "lh", "core", "cc_list", "cc_work", "rpy",
     "chol_step", "noise", "frames", "betas", "rotate", "translate", "average",
-    "commit", "energy_reduce", "cc_gather"};
+    "commit", "energy_reduce", "cc_gather", "chol_trail", "chol_diag"};
 
 }  // namespace
 
@@ -505,7 +505,11 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     c->chol.m = 3 * c->n_hi;
     c->chol.mpad = ((c->chol.m + 63) / 64) * 64;
     c->chol.ld = ((3 * nhi_cap + 63) / 64) * 64;
+    if (o.chol_mode != BD_CHOL_FP64 && o.chol_mode != BD_CHOL_FP32 && o.chol_mode != BD_CHOL_BF16)
+        return bail(fail(BD_E_ARG, "unsupported chol_mode %d", o.chol_mode));
     c->chol.mode = o.chol_mode;
+    c->chol.panel_tiles = 0;
+    if (const char *ev = getenv("BD_PANEL_TILES")) c->chol.panel_tiles = atoi(ev);
     {
         int ncu = 256;
         hipDeviceProp_t prop;
@@ -865,7 +869,8 @@ int bd_noise_displacement(bd_context *c, const double *p, double *rd) {
 
 int bd_set_chol_mode(bd_context *c, int mode) {
     if (check(c)) return BD_E_ARG;
-    if (mode != BD_CHOL_FP64 && mode != BD_CHOL_FP32) return fail(BD_E_ARG, "unsupported chol_mode %d", mode);
+    if (mode != BD_CHOL_FP64 && mode != BD_CHOL_FP32 && mode != BD_CHOL_BF16)
+        return fail(BD_E_ARG, "unsupported chol_mode %d", mode);
     if (mode != c->chol.mode) {
         HIPCHK(hipStreamSynchronize(c->stream));
         drop_graphs(c);
@@ -873,6 +878,25 @@ int bd_set_chol_mode(bd_context *c, int mode) {
         c->rp_count = 0;
     }
     c->chol.mode = mode;
+    return BD_OK;
+}
+
+int bd_set_chol_schedule(bd_context *c, int panel_tiles) {
+    if (check(c)) return BD_E_ARG;
+    if (panel_tiles < -1 || panel_tiles == 1) return fail(BD_E_ARG, "panel_tiles must be -1, 0 or >= 2 (got %d)", panel_tiles);
+    if (panel_tiles != c->chol.panel_tiles) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        drop_graphs(c);
+        c->factor_fresh = false;
+        c->rp_count = 0;
+    }
+    c->chol.panel_tiles = panel_tiles;
+    return BD_OK;
+}
+
+int bd_get_chol_schedule(bd_context *c, int *panel_tiles_used) {
+    if (check(c) || !panel_tiles_used) return fail(BD_E_ARG, "null argument");
+    *panel_tiles_used = chol_panel_tiles(c->chol);
     return BD_OK;
 }
 

@@ -28,7 +28,7 @@ void launch_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, h
 // (phase 0 before, 1 after); ids are the K_* values below.
 enum KernelId { K_SITES, K_MECH, K_LINKER, K_TAIL, K_LH, K_CORE, K_CCLIST, K_CCWORK,
                 K_RPY, K_CHOL_STEP, K_NOISE, K_FRAMES, K_BETAS, K_ROTATE, K_TRANSLATE,
-                K_AVG, K_COMMIT, K_EREDUCE, K_CCGATHER, K_COUNT };
+                K_AVG, K_COMMIT, K_EREDUCE, K_CCGATHER, K_CHOL_TRAIL, K_CHOL_DIAG, K_COUNT };
 typedef void (*MarkFn)(void *, int, int);
 // Extra streams of a force pass: after the charge sites, the tail, the
 // linker-histone (then, once mechanics is done, linker) and the core-core
@@ -73,7 +73,12 @@ struct CholWork {
     // two-launch form (k_panel + k_update): about two rounds of workgroups
     // at one per CU
     int split_tiles;
+    // super-panel width in block columns: 0 chooses (8 from 64 block
+    // columns up, none below), -1 never, k >= 2 always k
+    int panel_tiles;
 };
+// super-panel width the factor of c uses (0: one block column at a time)
+int chol_panel_tiles(const CholWork &c);
 // D into c.A; also clears the negative-pivot count of the factor that
 // follows, and with factor_first factors and inverts the first diagonal tile
 void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, const CholWork &c,
@@ -83,6 +88,9 @@ void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, co
 // (after the last one when gate_kb is past the end)
 void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark = nullptr, void *mark_arg = nullptr,
                      hipEvent_t gate = nullptr, int gate_kb = -1);
+// factor and inverse of the diagonal tile d alone (clear: restart the
+// negative-pivot count first)
+void launch_diag(const CholWork &c, int d, bool clear, hipStream_t q);
 void launch_noise(const Stat &s, int n_hi, int n_D3, const CholWork &c, const double *p,
                   const double *rad_all, double *rd, hipStream_t q);
 

@@ -82,6 +82,8 @@ API = {
     "bd_get_tensor": ([C.c_void_p, _dp], C.c_int),
     "bd_noise_displacement": ([C.c_void_p, _dp, _dp], C.c_int),
     "bd_set_chol_mode": ([C.c_void_p, C.c_int], C.c_int),
+    "bd_set_chol_schedule": ([C.c_void_p, C.c_int], C.c_int),
+    "bd_get_chol_schedule": ([C.c_void_p, _ip], C.c_int),
     "bd_set_frequency_rp": ([C.c_void_p, C.c_int], C.c_int),
     "bd_run_steps": ([C.c_void_p, C.c_int, _dp, _dp, _dp], C.c_int),
     "bd_set_timing": ([C.c_void_p, C.c_int], C.c_int),
@@ -332,6 +334,17 @@ class BDEngine:
 
     def set_chol_mode(self, mode: int):
         _check(self.lib.bd_set_chol_mode(self.h, mode), "bd_set_chol_mode")
+
+    def set_chol_schedule(self, panel_tiles: int):
+        """Super-panel width of the factor in 64-column block columns: 0
+        chooses by size, -1 one block column per launch, k >= 2 always k."""
+        _check(self.lib.bd_set_chol_schedule(self.h, int(panel_tiles)), "bd_set_chol_schedule")
+
+    def chol_schedule(self) -> int:
+        """Super-panel width in use (0: one block column per launch)."""
+        v = C.c_int(0)
+        _check(self.lib.bd_get_chol_schedule(self.h, C.byref(v)), "bd_get_chol_schedule")
+        return v.value
 
     def set_frequency_rp(self, frequency_rp: int):
         """Rebuild D and L every `frequency_rp` steps (1: every step)."""

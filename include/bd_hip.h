@@ -61,7 +61,9 @@ typedef struct bd_system {
 /* Cholesky arithmetic for the factor of D (the noise B = L). */
 #define BD_CHOL_FP64 0      /* everything FP64 (FP64 MFMA updates)              */
 #define BD_CHOL_FP32 1      /* trailing updates FP32 MFMA, FP32 accumulation    */
-#define BD_CHOL_BF16 2      /* trailing updates BF16 MFMA, FP32 accumulation    */
+#define BD_CHOL_BF16 2      /* trailing updates as three-slice BF16 MFMA products
+                               (each operand split exactly into three BF16
+                               numbers, six products), FP32 accumulation   */
 
 typedef struct bd_options {
     int device;             /* HIP device ordinal                               */
@@ -151,6 +153,19 @@ int bd_get_tensor(bd_context *ctx, double *D);
 /* rd = s2dt * L * p for the current factor (rd_cal, func_cuda.cu:634-647). */
 int bd_noise_displacement(bd_context *ctx, const double *p, double *rd);
 int bd_set_chol_mode(bd_context *ctx, int chol_mode);
+
+/* Schedule of the blocked factor (no counterpart in the reference, which
+ * calls cusolverDnDpotrf, func_cuda.cu:3526-3529).  panel_tiles = 0 (the
+ * default) chooses: super-panels of 8 block columns of 64 from 64 block
+ * columns (D of 4096 rows) up, one block column per launch below; -1 always
+ * one block column per launch; k >= 2 super-panels of k block columns
+ * whenever the matrix has more than k.  In a super-panel the panel columns
+ * are factored with their updates restricted to the panel, then the
+ * trailing matrix takes the panel's K = 64k update in one pass.  The
+ * environment variable BD_PANEL_TILES sets the initial value.
+ * bd_get_chol_schedule returns the super-panel width in use (0: none). */
+int bd_set_chol_schedule(bd_context *ctx, int panel_tiles);
+int bd_get_chol_schedule(bd_context *ctx, int *panel_tiles_used);
 
 /* frequency_RP (main.cpp:45, read from setup.txt): rebuild D and its factor
  * only every frequency_rp-th step and reuse L for rd = s2dt L p in between,

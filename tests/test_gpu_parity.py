@@ -331,3 +331,65 @@ def test_negative_pivot_factor():
         assert rel_err(D, z["D"]) <= 1e-14
         assert neg == int(z["neg"]) == 6
         assert rel_err(L, z["L"]) <= (TOL if mode == E.CHOL_FP64 else 1e-5)
+
+
+@pytest.mark.parametrize("panel", [-1, 2, 3, 8])
+def test_super_panel_schedule_factor(panel):
+    """The super-panel schedule (panel columns factored inside the panel,
+    one K = 64k trailing pass per panel: k_step restricted to the panel,
+    k_panel, k_trail, k_diag) against the oracle's factor, on the 6-core
+    deck with the full hydrodynamic set (D 1404 x 1404, 22 block columns:
+    several whole panels and a ragged last one), FP64; then the FP32 and
+    three-slice BF16 trailing updates against the FP64 factor."""
+    fib, st = deck("small", hi_all=1)
+    orc = O.Oracle(fib)
+    fs = orc.prepare(st)
+    r_all = np.concatenate([fs.view("r"), fs.view("r_t"), fs.view("r_lh")])[:3 * fib.n_hi]
+    D0, L0, neg0 = orc.diffusion_tensor(r_all, fib.rad_all()[:fib.n_hi])
+    Ls = {}
+    for mode in (E.CHOL_FP64, E.CHOL_FP32, E.CHOL_BF16):
+        with E.BDEngine(fib, chol_mode=mode, use_graph=False) as eng:
+            eng.set_chol_schedule(panel)
+            assert eng.chol_schedule() == (0 if panel < 0 else panel)
+            sync_engine(eng, fs)
+            eng.factor()
+            Ls[mode], neg = eng.get_factor()
+            assert neg == neg0 == 0
+    assert rel_err(Ls[E.CHOL_FP64], L0) <= TOL
+    for mode in (E.CHOL_FP32, E.CHOL_BF16):
+        e = np.linalg.norm(Ls[mode] @ Ls[mode].T - D0) / np.linalg.norm(D0)
+        print(f"mode {mode} panel {panel}: |LL^T - D| / |D| = {e:.2e}, "
+              f"|L - L64| / |L64| = {rel_err(Ls[mode], L0):.2e}")
+        assert e <= 1e-6 and rel_err(Ls[mode], L0) <= 1e-5
+
+
+def _mixed_step_errors(fib, st, rr, p, modes):
+    """Displacement of one step and ||L L^T - D|| / ||D|| for each factor
+    mode, from the same state and noise."""
+    res = {}
+    for mode in modes:
+        with E.BDEngine(fib, chol_mode=mode) as eng:
+            eng.set_state(st, rebuild=True)
+            x0 = eng.get_state()
+            eng.step(rr, p)
+            x1 = eng.get_state()
+        res[mode] = np.concatenate([x1.r - x0.r, x1.r_t - x0.r_t, x1.r_lh - x0.r_lh])
+    return res
+
+
+def test_bf16_factor_and_displacement_cfg2():
+    """Three-slice BF16 trailing updates at the 100-nucleosome fiber: the
+    factor within 1e-6 of D and the step's displacement within 1e-6 of the
+    FP64 step (same state and noise)."""
+    fib, st = Fb.make_config(2)
+    rr, p = noise(fib)
+    d = _mixed_step_errors(fib, st, rr, p, (E.CHOL_FP64, E.CHOL_BF16))
+    rel = np.linalg.norm(d[E.CHOL_BF16] - d[E.CHOL_FP64]) / np.linalg.norm(d[E.CHOL_FP64])
+    with E.BDEngine(fib, chol_mode=E.CHOL_BF16) as eng:
+        eng.set_state(st, rebuild=True)
+        D = eng.get_tensor()
+        eng.factor()
+        L, neg = eng.get_factor()
+    err = np.linalg.norm(L @ L.T - D) / np.linalg.norm(D)
+    print(f"cfg2 bf16: displacement {rel:.2e}, |LL^T - D| / |D| = {err:.2e}")
+    assert neg == 0 and err < 1e-6 and rel < 1e-6
