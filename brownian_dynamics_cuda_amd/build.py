@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbdhip.so")
 SOURCES = ["bd_forces.hip", "bd_integrate.hip", "bd_chol.hip", "bd_engine.hip", "bd_legacy.hip",
-           "bd_rng.hip"]
+           "bd_rng.hip", "bd_potrf.hip", "bd_adjoint.hip"]
 
 # -packed-fp32-ops: no v_pk_*_f32 (packed FP32) instructions in any kernel.
 HIPFLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",

@@ -22,6 +22,8 @@
 // (k_trail: each tile accumulates K = 64 NB in registers before its one
 // read-modify-write), so the trailing matrix streams through HBM once per
 // NB block columns rather than once per block column.
+#include <algorithm>
+
 #include "bd_dev.h"
 #include "bd_kernels.h"
 
@@ -1106,6 +1108,44 @@ void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_a
 
 void launch_diag(const CholWork &c, int d, bool clear, hipStream_t q) {
     k_diag<<<1, 256, 0, q>>>(c.A, c.ld, d, c.Linv, c.neg, clear ? 1 : 0);
+}
+
+// A caller's matrix into the factor's storage: the lower triangle of D
+// (row-major, leading dimension ldd), zeros above the diagonal and the
+// identity on the padding rows; and back: the factor, unparked (the
+// off-diagonal tile (ib, kb) lives in the upper block (kb, ib)), into L
+// with zeros above the diagonal.
+__global__ void k_pack(const double *__restrict__ D, int ldd, int m, double *__restrict__ A, int ld, int mpad) {
+    const long long total = (long long)mpad * mpad;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(e / mpad), c = (int)(e % mpad);
+        double v = 0.0;
+        if (c <= r) v = (r < m) ? D[(size_t)r * ldd + c] : (r == c ? 1.0 : 0.0);
+        A[(size_t)r * ld + c] = v;
+    }
+}
+__global__ void k_unpark(const double *__restrict__ A, int ld, int m, double *__restrict__ L, int ldl) {
+    const long long total = (long long)m * m;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(e / m), j = (int)(e % m), ib = i / TB, kb = j / TB;
+        double v = 0.0;
+        if (kb < ib) v = A[(size_t)(kb * TB + i % TB) * ld + ib * TB + j % TB];
+        else if (j <= i) v = A[(size_t)i * ld + j];
+        L[(size_t)i * ldl + j] = v;
+    }
+}
+
+void launch_pack(const double *D, int ldd, const CholWork &c, hipStream_t q) {
+    const long long total = (long long)c.mpad * c.mpad;
+    const int grid = (int)std::min<long long>((total + 255) / 256, 4096);
+    k_pack<<<grid, 256, 0, q>>>(D, ldd, c.m, c.A, c.ld, c.mpad);
+}
+void launch_unpark(const CholWork &c, double *L, int ldl, hipStream_t q) {
+    const long long total = (long long)c.m * c.m;
+    const int grid = (int)std::max<long long>(1, std::min<long long>((total + 255) / 256, 4096));
+    k_unpark<<<grid, 256, 0, q>>>(c.A, c.ld, c.m, L, ldl);
 }
 
 void launch_noise(const Stat &s, int n_hi, int n_D3, const CholWork &c, const double *p, const double *rad_all,

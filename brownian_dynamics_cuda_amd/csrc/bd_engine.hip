@@ -60,6 +60,10 @@ const char *kKernelNames[K_COUNT] = {
 
 }  // namespace
 
+namespace bd {
+void set_last_error(const char *msg) { g_err = msg; }
+}  // namespace bd
+
 struct bd_context {
     int device = 0;
     hipStream_t stream = nullptr;

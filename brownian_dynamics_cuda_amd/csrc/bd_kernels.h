@@ -4,6 +4,9 @@
 
 namespace bd {
 
+// the message bd_last_error() returns (per host thread)
+void set_last_error(const char *msg);
+
 // Scratch of the core-core charge sums.
 struct ForceScratch {
     unsigned char *cc_valid;  // n_c * n_c
@@ -91,6 +94,10 @@ void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark = nullptr, vo
 // factor and inverse of the diagonal tile d alone (clear: restart the
 // negative-pivot count first)
 void launch_diag(const CholWork &c, int d, bool clear, hipStream_t q);
+// a caller's matrix (lower triangle of row-major D) into c.A with the
+// identity padding; the factor in c.A back out as row-major lower L
+void launch_pack(const double *D, int ldd, const CholWork &c, hipStream_t q);
+void launch_unpark(const CholWork &c, double *L, int ldl, hipStream_t q);
 void launch_noise(const Stat &s, int n_hi, int n_D3, const CholWork &c, const double *p,
                   const double *rad_all, double *rd, hipStream_t q);
 

@@ -83,6 +83,12 @@ API = {
     "bd_noise_displacement": ([C.c_void_p, _dp, _dp], C.c_int),
     "bd_set_chol_mode": ([C.c_void_p, C.c_int], C.c_int),
     "bd_set_chol_schedule": ([C.c_void_p, C.c_int], C.c_int),
+    "bd_cc_forward": ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+                      + [C.c_double] * 4 + [C.c_void_p] * 3, C.c_int),
+    "bd_cc_backward": ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+                       + [C.c_double] * 4 + [C.c_void_p] * 5, C.c_int),
+    "bd_cholesky": ([C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, _ip, C.c_void_p],
+                    C.c_int),
     "bd_get_chol_schedule": ([C.c_void_p, _ip], C.c_int),
     "bd_set_frequency_rp": ([C.c_void_p, C.c_int], C.c_int),
     "bd_run_steps": ([C.c_void_p, C.c_int, _dp, _dp, _dp], C.c_int),
@@ -404,6 +410,24 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+def cholesky_device(D: np.ndarray, mode: int = CHOL_FP64, entry: str = "bd_cholesky"):
+    """L and the negative-pivot count of a symmetric matrix, factored on the
+    GPU from a device copy of D through the C-ABI entry point `entry`."""
+    lib = load_library()
+    D = np.ascontiguousarray(D, np.float64)
+    m = D.shape[0]
+    dD, dL = DeviceBuffer(D.nbytes), DeviceBuffer(D.nbytes)
+    try:
+        dD.upload(D)
+        neg = C.c_int(-1)
+        _check(getattr(lib, entry)(m, dD.ptr, m, dL.ptr, m, mode, C.byref(neg), None), entry)
+        L = dL.download((m, m))
+    finally:
+        dD.free()
+        dL.free()
+    return L, neg.value
 
 
 class HostRng:

@@ -167,6 +167,43 @@ int bd_set_chol_mode(bd_context *ctx, int chol_mode);
 int bd_set_chol_schedule(bd_context *ctx, int panel_tiles);
 int bd_get_chol_schedule(bd_context *ctx, int *panel_tiles_used);
 
+/* Cholesky factor of a caller's symmetric matrix in device memory with the
+ * library's blocked factor: replaces cusolverDnDpotrf +
+ * Cholesky_Decomposition_mod (func_cuda.cu:3513-3534, 614-630), with the CPU
+ * rule of func.cpp:806-836 on a negative pivot (|f| taken and counted;
+ * cuSOLVER instead stops and the reference maps the NaNs to 0).
+ * D: m x m row-major, only its lower triangle read (ldd >= m); L: m x m
+ * row-major (ldl >= m), the lower factor with zeros above the diagonal; D
+ * and L may be the same buffer when ldd == ldl.  mode: BD_CHOL_*.  neg: host
+ * int for the count of negative pivots (may be NULL).  stream: a hipStream_t
+ * (NULL: the default stream); the call returns once the factor is done.
+ * The workspace, (64 ceil(m/64))^2 + 4096 ceil(m/64) doubles, is allocated
+ * and freed by the call. */
+int bd_cholesky(int m, const double *D, int ldd, double *L, int ldl, int mode, int *neg,
+                void *stream);
+
+/* Core-core charge term of the force pass (func.cpp:1489-1566) and its
+ * adjoint, for the differentiable trajectory (a torch.autograd.Function on
+ * the Python side; no counterpart in the reference).  All arrays are device
+ * pointers of one arithmetic: double when fp64 != 0, float otherwise.
+ *   Q[n_c][Nq][3]  world positions of every core's charges
+ *   q[Nq]          charges (one template shared by all cores)
+ *   nbr_off[n_c+1], nbr[]: for each core, the cores it interacts with
+ *                  (both directions: j in i's list iff i in j's)
+ *   ke, kappa, kex, sigma: k_e, Debye kappa, k_ex, sigma_core-core
+ *   f[n_c][Nq][3]  force on each charge (written)
+ *   e[n_c]         half of the energy of each core's pairs (written)
+ * bd_cc_backward takes gf = dLoss/df and ge = dLoss/de (same shapes) and
+ * writes gQ = dLoss/dQ and gq_sites[n_c][Nq] = dLoss/dq per charge site
+ * (the template's gradient is the sum over cores).  Asynchronous on
+ * stream (a hipStream_t; NULL: the default stream). */
+int bd_cc_forward(int fp64, int n_c, int Nq, const void *Q, const void *q, const int *nbr_off,
+                  const int *nbr, double ke, double kappa, double kex, double sigma, void *f,
+                  void *e, void *stream);
+int bd_cc_backward(int fp64, int n_c, int Nq, const void *Q, const void *q, const int *nbr_off,
+                   const int *nbr, double ke, double kappa, double kex, double sigma,
+                   const void *gf, const void *ge, void *gQ, void *gq_sites, void *stream);
+
 /* frequency_RP (main.cpp:45, read from setup.txt): rebuild D and its factor
  * only every frequency_rp-th step and reuse L for rd = s2dt L p in between,
  * the gate that func_cuda.cu:3504 has commented out.  1 (the default, and the

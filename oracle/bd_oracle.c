@@ -1213,8 +1213,8 @@ static void translate(const or_system *sys, const double *D, int m_hi, const dou
     }
 }
 
-int or_step(const or_system *sys, or_state *st, const double *rr, const double *p,
-            double *energy, or_step_out *out) {
+static int step_impl(const or_system *sys, or_state *st, const double *rr, const double *p,
+                     const double *L_reuse, double *energy, or_step_out *out) {
     const int n = sys->n, n3 = 3 * n, nt3 = 3 * sys->n_tail, nl3 = 3 * sys->n_LH;
     const int n_D = n + sys->n_tail + sys->n_LH, n_D3 = 3 * n_D;
     const int n_hi = sys->hi_all ? n_D : n, m = 3 * n_hi;
@@ -1238,7 +1238,9 @@ int or_step(const or_system *sys, or_state *st, const double *rr, const double *
     double *D = malloc(sizeof(double) * (size_t)m * m);
     double *L = malloc(sizeof(double) * (size_t)m * m);
     or_diffusion_tensor(n_hi, r_all, rad_all, sys->a1, sys->a2, D);
-    int neg = or_cholesky(m, D, L);
+    int neg = 0;
+    if (L_reuse) memcpy(L, L_reuse, sizeof(double) * (size_t)m * m);
+    else neg = or_cholesky(m, D, L);
 
     /* rd = s2dt * L * p (rd_cal, func_cuda.cu:634-647); beads outside the
      * HI set: s2dt * sqrt(a1/rad) * p */
@@ -1336,6 +1338,16 @@ int or_step(const or_system *sys, or_state *st, const double *rr, const double *
     free(pr.alpha_p); free(pr.beta_p); free(pr.gamma_p); free(pr.r_t); free(pr.beta_t);
     free(pr.r_lh); free(pr.beta_lh);
     return ret;
+}
+
+int or_step(const or_system *sys, or_state *st, const double *rr, const double *p,
+            double *energy, or_step_out *out) {
+    return step_impl(sys, st, rr, p, NULL, energy, out);
+}
+
+int or_step_reuse(const or_system *sys, or_state *st, const double *rr, const double *p,
+                  const double *L_reuse, double *energy, or_step_out *out) {
+    return step_impl(sys, st, rr, p, L_reuse, energy, out);
 }
 
 /* ------------------------------------------------------------------ */

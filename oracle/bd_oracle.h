@@ -117,6 +117,12 @@ void or_rotate(int n, const double *a, const double *b, const double *c, double 
 /* One Ermak-McCammon predictor-corrector step (in place on st). */
 int or_step(const or_system *sys, or_state *st, const double *rr, const double *p,
             double *energy, or_step_out *out);
+/* The same step with the factor of an earlier step (frequency_RP,
+ * main.cpp:45; the rebuild gate that func_cuda.cu:3504 has commented out):
+ * rd = s2dt L_reuse p, while D, which the drift uses, is assembled at the
+ * current state as every step does (L_reuse: n_hi3 x n_hi3 row-major). */
+int or_step_reuse(const or_system *sys, or_state *st, const double *rr, const double *p,
+                  const double *L_reuse, double *energy, or_step_out *out);
 
 /* Host RNG: MT19937 (mt.cpp:48-141) and the polar normal sampler
  * (utilities.cpp:304-336) with its cached second value. */

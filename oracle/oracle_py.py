@@ -173,6 +173,8 @@ def _setup(lib, prefix):
         lib.or_forces.argtypes = [sp, stp, _dp, _dp, _dp, _dp, _dp]
         lib.or_step.argtypes = [sp, stp, _dp, _dp, _dp, C.POINTER(CStepOut)]
         lib.or_step.restype = C.c_int
+        lib.or_step_reuse.argtypes = [sp, stp, _dp, _dp, _dp, _dp, C.POINTER(CStepOut)]
+        lib.or_step_reuse.restype = C.c_int
         lib.or_diffusion_tensor.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, _dp]
         lib.or_cholesky.argtypes = [C.c_int, _dp, _dp]
         lib.or_cholesky.restype = C.c_int
@@ -317,7 +319,9 @@ class Oracle:
                                       C.byref(E))
         return dict(F=F, T=T, tF=tF[:3 * f.n_tail], lhF=lF[:3 * f.n_LH], E=E.value)
 
-    def step(self, fs: FullState, rr, p, want=False):
+    def step(self, fs: FullState, rr, p, want=False, L_reuse=None):
+        """One step; with L_reuse (oracle only) the noise uses that factor
+        of an earlier step instead of the factor of the current D."""
         rr = _d(rr)
         p = _d(p)
         E = C.c_double(0.0)
@@ -335,10 +339,17 @@ class Oracle:
                                   ("lhF0", 3 * f.n_LH + 3), ("lhF1", 3 * f.n_LH + 3)):
                         out[k] = np.zeros(max(sz, 1))
                         setattr(o, k, _ptr(out[k]))
-                self.lib.or_step(C.byref(self.sb.c), C.byref(fs.cstruct()), _ptr(rr), _ptr(p),
-                                 C.byref(E), C.byref(o))
+                if L_reuse is None:
+                    self.lib.or_step(C.byref(self.sb.c), C.byref(fs.cstruct()), _ptr(rr), _ptr(p),
+                                     C.byref(E), C.byref(o))
+                else:
+                    Lr = _d(L_reuse).reshape(-1)
+                    self.lib.or_step_reuse(C.byref(self.sb.c), C.byref(fs.cstruct()), _ptr(rr),
+                                           _ptr(p), _ptr(Lr), C.byref(E), C.byref(o))
                 out["neg_pivots"] = o.neg_pivots
             else:
+                if L_reuse is not None:
+                    raise ValueError("L_reuse is an oracle-only mode")
                 self.lib.ref_step(C.byref(self.sb.c), C.byref(fs.cstruct()), _ptr(rr), _ptr(p),
                                   C.byref(E))
         out["E"] = E.value

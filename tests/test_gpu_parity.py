@@ -393,3 +393,31 @@ def test_bf16_factor_and_displacement_cfg2():
     err = np.linalg.norm(L @ L.T - D) / np.linalg.norm(D)
     print(f"cfg2 bf16: displacement {rel:.2e}, |LL^T - D| / |D| = {err:.2e}")
     assert neg == 0 and err < 1e-6 and rel < 1e-6
+
+
+@pytest.mark.parametrize("hi", [0, 1])
+def test_frequency_rp_against_oracle_reuse(hi):
+    """frequency_RP = 3 (bd_set_frequency_rp): steps 0 and 3 rebuild D and
+    L, the others reuse the factor of the last rebuild for rd, against the
+    oracle's reuse mode (or_step_reuse) over six free-running steps."""
+    fib, st = deck("small", hi_all=hi)
+    orc = O.Oracle(fib)
+    fs = orc.prepare(st)
+    g = O.Rng(41)
+    nD3 = 3 * (fib.n + fib.n_tail + fib.n_LH)
+    Lk = None
+    with E.BDEngine(fib) as eng:
+        eng.set_state(st, rebuild=True)
+        eng.set_frequency_rp(3)
+        for s in range(6):
+            rr, p = g.step_noise(s, fib.r_var(), nD3)
+            e = eng.step(rr, p)
+            if s % 3 == 0:
+                out = orc.step(fs, rr, p, want=True)
+                Lk = out["L"]
+            else:
+                out = orc.step(fs, rr, p, want=True, L_reuse=Lk)
+            assert e == pytest.approx(out["E"], rel=1e-9)
+            got = eng.get_state()
+            for k in ("r", "a", "b", "c", "r_t", "r_lh"):
+                assert rel_err(getattr(got, k), fs.view(k)) <= 1e-9, (s, k)

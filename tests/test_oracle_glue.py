@@ -181,3 +181,37 @@ def test_rotation_zeta_and_linker_component():
             assert rv[3 * i] == pytest.approx(2 * fib.phys.time_step * kT / ZETA_LINKER, rel=1e-15)
             assert rv[3 * i + 1] == 0 and rv[3 * i + 2] == 0
     assert math.isclose(ZETA_CORE / ZETA_LINKER, 2 * 125.0 / (R_H * R_H * LO))
+
+
+def test_reuse_factor_mode():
+    """or_step_reuse (frequency_RP): with the step's own factor it is the
+    step; with the factor of an earlier state, rd = s2dt L_old p while D
+    (for the drift) is that of the current state."""
+    fib, st = deck("small", hi_all=1)
+    orc = O.Oracle(fib)
+    fs0 = orc.prepare(st)
+    g = O.Rng(31)
+    nD3 = 3 * (fib.n + fib.n_tail + fib.n_LH)
+    rr0, p0 = g.step_noise(0, fib.r_var(), nD3)
+    rr1, p1 = g.step_noise(1, fib.r_var(), nD3)
+    a = fs0.copy()
+    out0 = orc.step(a, rr0, p0, want=True)
+    b = fs0.copy()
+    outb = orc.step(b, rr0, p0, want=True, L_reuse=out0["L"])
+    assert outb["E"] == out0["E"]
+    for k in ("r", "a", "b", "c", "r_t", "r_lh"):
+        assert np.array_equal(a.view(k), b.view(k)), k
+    m = 3 * fib.n_hi
+    c = a.copy()
+    out1 = orc.step(c, rr1, p1, want=True, L_reuse=out0["L"])
+    L0 = out0["L"].reshape(m, m)
+    assert np.array_equal(out1["L"].reshape(m, m), L0)
+    assert rel_err(out1["rd"][:m], fib.phys.s2dt * (L0 @ p1[:m])) <= 1e-14
+    r_all = np.concatenate([a.view("r"), a.view("r_t"), a.view("r_lh")])[:m]
+    D1, _, _ = orc.diffusion_tensor(r_all, fib.rad_all()[:fib.n_hi])
+    assert np.array_equal(out1["D"].reshape(m, m), D1)
+    # the fresh step from the same state differs only through rd
+    d = a.copy()
+    out1f = orc.step(d, rr1, p1, want=True)
+    assert not np.array_equal(out1f["rd"], out1["rd"])
+    assert np.array_equal(out1f["D"], out1["D"])
