@@ -48,9 +48,13 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the timed steps, run the library's streaming copy kernel "
                          "once over MIB MiB (calibrates HBM traffic counters)")
-    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles",
-                                                           "hbm_traffic_cfg2.txt"),
-                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the roofline traffic")
+    ap.add_argument("--traffic-file", default=None,
+                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the roofline traffic "
+                         "(default: profiles/hbm_traffic_cfg<config>.txt; traffic is null "
+                         "when no summary exists for the configuration)")
+    ap.add_argument("--e2e-steps", type=int, default=200,
+                    help="steps of the end-to-end rate with the host noise draw overlapped "
+                         "(0: skip)")
     return ap.parse_args()
 
 
@@ -151,27 +155,78 @@ def main():
     dominant = max(per_step, key=per_step.get)
 
     # roofline of the dominant kernel
+    tot_ms, launches = kt[dominant]
+    avg_s = tot_ms / launches / 1e3
     if dominant in ("chol_step", "potrf"):
         # the factorisation's algorithmic FLOPs, n^3/3, spread over the
         # block-column launches (one per 64 columns after the first)
-        tot_ms, launches = kt[dominant]
-        avg_s = tot_ms / launches / 1e3
         per_launch = chol_flops(n_hi3) / (launches / args.timing_steps)
         roof = dict(bound="mfma", achieved=per_launch / avg_s / 1e12, peak=PEAK_FP64 / 1e12,
                     unit="TFLOP/s", kernel=dominant, pipe="FP64 MFMA",
                     flop_per_launch=per_launch, avg_launch_us=avg_s * 1e6)
+    elif dominant == "chol_trail":
+        # super-panel trailing updates: the lower triangle of each trailing
+        # matrix (m_s rows) takes a rank-K update, K = 64 NB: K m_s (m_s + 1)
+        # FLOPs; averaged over the launches of one factorisation
+        nb = eng.chol_schedule()
+        nt = (n_hi3 + 63) // 64
+        fl = []
+        k0 = 0
+        while nb and k0 + nb < nt:
+            ms = n_hi3 - 64 * (k0 + nb)
+            fl.append(64.0 * nb * ms * (ms + 1))
+            k0 += nb
+        per_launch = sum(fl) / len(fl)
+        roof = dict(bound="mfma", achieved=per_launch / avg_s / 1e12, peak=PEAK_FP64 / 1e12,
+                    unit="TFLOP/s", kernel=dominant, pipe="FP64 MFMA",
+                    flop_per_launch=per_launch, avg_launch_us=avg_s * 1e6, panel_tiles=nb)
     else:
         # Debye-Hueckel charge sums: 38 FLOPs per charge pair, FP64 on the
         # vector ALUs, whose MI355X peak equals the FP64 matrix peak
         pair_flops = 38.0
-        tot_ms, launches = kt[dominant]
-        avg_s = tot_ms / launches / 1e3
         per_launch = counts["core_charge_pairs"] * pair_flops
         roof = dict(bound="mfma", achieved=per_launch / avg_s / 1e12, peak=PEAK_FP64 / 1e12,
                     unit="TFLOP/s", kernel=dominant, pipe="FP64 VALU",
                     flop_per_launch=per_launch, avg_launch_us=avg_s * 1e6)
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = read_traffic(args.traffic_file, dominant)
+    tfile = args.traffic_file or os.path.join(ROOT, "profiles", f"hbm_traffic_cfg{cfg}.txt")
+    roof["traffic"] = read_traffic(tfile, dominant)
+
+    # end to end: the host draws each batch's noise (MT19937 + polar normals,
+    # main.cpp:479-486) on a thread while the GPU runs the previous batch
+    # (bd_run_steps: upload, steps, energies back), as the host driver does
+    e2e = None
+    if args.e2e_steps > 0:
+        import threading
+        bsz = 50
+        nb_ = max(1, args.e2e_steps // bsz)
+        rng2 = E.HostRng(12345)
+        rv = fib.r_var()
+
+        def draw(step0, out):
+            a = np.zeros((bsz, n3))
+            b = np.zeros((bsz, nD3))
+            for k in range(bsz):
+                a[k], b[k] = rng2.step_noise(step0 + k, rv, nD3)
+            out.append((a, b))
+
+        eng.set_state(st, rebuild=True)
+        eng.synchronize()
+        t1 = time.perf_counter()
+        cur = []
+        draw(0, cur)
+        for bi in range(nb_):
+            nxt = []
+            th = threading.Thread(target=draw, args=((bi + 1) * bsz, nxt)) if bi + 1 < nb_ else None
+            if th:
+                th.start()
+            eng.run_steps(*cur[0])
+            if th:
+                th.join()
+            cur = nxt
+        de = time.perf_counter() - t1
+        e2e = dict(steps_per_s=nb_ * bsz / de, steps=nb_ * bsz, batch=bsz,
+                   note="host noise drawn on one host thread, overlapped with the previous batch")
 
     # mixed-precision factors: same state and noise, one step each way;
     # displacement error against FP64, factor error ||L L^T - D|| / ||D||
@@ -270,6 +325,7 @@ def main():
         "work": counts,
         "finite": finite,
         "pcie_inclusive_steps_per_s": host_io,
+        "end_to_end": e2e,
         "noise_gen_s": round(t_noise, 2),
     }
     if cpu:
