@@ -9,6 +9,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "../../include/bd_hip.h"
@@ -87,6 +88,37 @@ int run_factor(int m, const double *D, int ldd, double *L, int ldl, int mode, in
     return BD_OK;
 }
 
+// Workspace of bd_cholesky_noalloc: a static device array of the library,
+// sized for matrices of up to kNoallocMaxM rows (33 block columns; the
+// 2082-row tensor of the 100-nucleosome fiber fits): 36.8 MB that every GPU
+// the library runs on holds until the process ends.  Host code never uses
+// the array's name as an address: the device address comes from
+// hipGetSymbolAddress, once per device, and is kept in g_scratch.
+constexpr int kNoallocMaxM = BD_CHOLESKY_NOALLOC_MAX_M;
+constexpr size_t kNoallocDoubles =
+    (size_t)kNoallocMaxM * kNoallocMaxM + (size_t)(kNoallocMaxM / 64) * 64 * 64 + 32;
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+int scratch_of_current_device(double **out) {
+    int dev = 0;
+    POTRF_CHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(BD_E_ARG, "device ordinal %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(BD_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch[dev];
+    return BD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -102,6 +134,18 @@ int bd_cholesky(int m, const double *D, int ldd, double *L, int ldl, int mode, i
     if (r) return r;
     if (e != hipSuccess) return fail(BD_E_HIP, "hipFree: %s", hipGetErrorString(e));
     return BD_OK;
+}
+
+int bd_cholesky_noalloc(int m, const double *D, int ldd, double *L, int ldl, int mode, int *neg, void *stream) {
+    int r = check_args(m, D, ldd, L, ldl, mode);
+    if (r) return r;
+    if (m > kNoallocMaxM)
+        return fail(BD_E_CAPACITY, "m=%d exceeds the static workspace of bd_cholesky_noalloc (at most %d rows)", m,
+                    kNoallocMaxM);
+    double *ws = nullptr;
+    r = scratch_of_current_device(&ws);
+    if (r) return r;
+    return run_factor(m, D, ldd, L, ldl, mode, neg, static_cast<hipStream_t>(stream), ws);
 }
 
 }  // extern "C"

@@ -89,6 +89,8 @@ API = {
                        + [C.c_double] * 4 + [C.c_void_p] * 5, C.c_int),
     "bd_cholesky": ([C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, _ip, C.c_void_p],
                     C.c_int),
+    "bd_cholesky_noalloc": ([C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, _ip,
+                             C.c_void_p], C.c_int),
     "bd_get_chol_schedule": ([C.c_void_p, _ip], C.c_int),
     "bd_set_frequency_rp": ([C.c_void_p, C.c_int], C.c_int),
     "bd_run_steps": ([C.c_void_p, C.c_int, _dp, _dp, _dp], C.c_int),

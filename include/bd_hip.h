@@ -182,6 +182,19 @@ int bd_get_chol_schedule(bd_context *ctx, int *panel_tiles_used);
 int bd_cholesky(int m, const double *D, int ldd, double *L, int ldl, int mode, int *neg,
                 void *stream);
 
+/* bd_cholesky without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The workspace is a
+ * static device array of the library, BD_CHOLESKY_NOALLOC_MAX_M rows at
+ * most (36.8 MB, held by every GPU the library runs on until the process
+ * ends); larger m returns BD_E_CAPACITY.  All calls on one GPU share that
+ * array: calls that overlap on one GPU (two host threads, or two streams)
+ * race and must be serialised by the caller.  Same arguments and results as
+ * bd_cholesky. */
+#define BD_CHOLESKY_NOALLOC_MAX_M 2112
+int bd_cholesky_noalloc(int m, const double *D, int ldd, double *L, int ldl, int mode, int *neg,
+                        void *stream);
+
 /* Core-core charge term of the force pass (func.cpp:1489-1566) and its
  * adjoint, for the differentiable trajectory (a torch.autograd.Function on
  * the Python side; no counterpart in the reference).  All arrays are device
