@@ -42,6 +42,9 @@ def parse():
                     help="steps timed through bd_step with host noise buffers (PCIe-inclusive rate)")
     ap.add_argument("--cpu-steps", type=int, default=2, help="reference CPU steps to time (0: skip)")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--chol", choices=("fp64", "fp32", "bf16"), default="fp64",
+                    help="factor arithmetic of the timed steps (the headline is fp64; the "
+                         "others are for profiling the mixed modes)")
     ap.add_argument("--eager", action="store_true",
                     help="launch the step's kernels one by one instead of replaying the "
                          "captured hipGraph (same kernels; for profilers)")
@@ -104,7 +107,8 @@ def main():
     t_noise = time.time() - t0
     n3, nD3 = rr.shape[1], p.shape[1]
 
-    eng = E.BDEngine(fib, chol_mode=E.CHOL_FP64, use_graph=not args.eager)
+    mode = {"fp64": E.CHOL_FP64, "fp32": E.CHOL_FP32, "bf16": E.CHOL_BF16}[args.chol]
+    eng = E.BDEngine(fib, chol_mode=mode, use_graph=not args.eager)
     eng.set_state(st, rebuild=True)
     d_rr = E.DeviceBuffer(rr.nbytes)
     d_p = E.DeviceBuffer(p.nbytes)
@@ -313,7 +317,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.chol == "fp64" else f"f64 (factor trailing updates {args.chol})",
         "data": "synthetic (seeded zigzag fiber, host MT19937 noise)",
         "config": {"workload": f"cfg{cfg}: {fib.n_c}-nucleosome fiber, {fib.n} cores+linker beads, "
                                f"{fib.n_tail} tail beads, {fib.n_LH} free LH beads, "

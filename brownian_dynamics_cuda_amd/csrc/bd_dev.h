@@ -81,6 +81,28 @@ struct Stat {
 constexpr int kChunk = 16;
 __host__ __device__ inline int nchunks(int n) { return (n + kChunk - 1) / kChunk; }
 
+// Cell grid of one partner set, rebuilt per pass on the GPU: cubic cells
+// of edge at least the 7 nm reach over the set's bounding box (the edge
+// grows when the box would need more than kCellMax cells); the members of
+// each cell are the partner indices in increasing order, cells in x-fastest
+// order, so a run of cells along x is one contiguous range of members.
+constexpr int kCellMax = 1 << 17;
+struct CellGrid {
+    // the set's points for the build, set per pass by launch_sites:
+    // point k is at pos + 3 idx[k] (idx NULL: pos + 3 k)
+    const double *pos;
+    const int *idx;
+    int n;
+    double *hdr;   // 4: origin x, y, z and 1 / edge
+    int *dim;      // 3: nx, ny, nz (0 for an empty set)
+    int *start;    // kCellMax + 1: first member of each cell
+    int *cursor;   // kCellMax: fill cursor of the build
+    int *members;  // partner indices, cell by cell
+    int *cid;      // cell of each partner
+};
+// partner sets of the culled loops
+enum { SET_TAIL = 0, SET_LH = 1, SET_HEAD = 2, SET_LINKER = 3, NSETS = 4 };
+
 struct Sites {
     double *Q;  // 3 * n_c * Nq world positions of core charges
     // 4 * n_c * Nq: per core charge {world position (3), charge}, the
@@ -91,6 +113,9 @@ struct Sites {
     // partners, rebuilt per pass: tail beads, free LH beads, head sites, and
     // linker beads in the order of the linker list
     double4 *ch_t, *ch_lh, *ch_g, *ch_l;
+    // the same partner sets as cell grids; use_cells selects them
+    CellGrid cg[NSETS];
+    int use_cells;
 };
 
 __device__ __forceinline__ double3 ld3(const double *p) { return make_double3(p[0], p[1], p[2]); }

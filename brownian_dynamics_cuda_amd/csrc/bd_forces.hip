@@ -225,6 +225,237 @@ __device__ __forceinline__ void for_near(const double4 *ch, int n, double3 p, do
     }
 }
 
+// --- cell grids of the partner sets (the same pair set, fewer candidates) ---
+__device__ __forceinline__ int set_size(const Stat &s, int set) {
+    return set == SET_TAIL ? s.n_tail : set == SET_LH ? s.n_LH : set == SET_HEAD ? s.n_lhc * s.n_lh_g : s.n_lnk;
+}
+__device__ __forceinline__ double3 grid_point(const CellGrid &cg, int k) {
+    return ld3(cg.pos + 3 * (cg.idx ? cg.idx[k] : k));
+}
+__device__ __forceinline__ int cell_of(double3 p, const double *o, double inv, int nx, int ny, int nz) {
+    const int ix = min(max((int)floor((p.x - o[0]) * inv), 0), nx - 1);
+    const int iy = min(max((int)floor((p.y - o[1]) * inv), 0), ny - 1);
+    const int iz = min(max((int)floor((p.z - o[2]) * inv), 0), nz - 1);
+    return (iz * ny + iy) * nx + ix;
+}
+
+// Exclusive scan of v over an NT-thread workgroup; returns this thread's
+// offset and the total in *tot.  Contains barriers.
+template <int NT>
+__device__ __forceinline__ int block_excl_scan(int v, int *wsum, int *tot) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int off = 0, t = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; i++) {
+        off += (i < w) ? wsum[i] : 0;
+        t += wsum[i];
+    }
+    *tot = t;
+    __syncthreads();
+    return off + x - v;
+}
+
+// One workgroup per partner set: bounding box, grid, counts, offsets and
+// the fill (atomic cursors; k_cell_sort orders each cell afterwards).
+constexpr int kCellThreads = 1024;
+__global__ void __launch_bounds__(kCellThreads) k_cells(Sites st) {
+    __shared__ double red[6 * (kCellThreads / 64)];
+    __shared__ double s_hdr[4];
+    __shared__ int s_dim[3], s_wsum[kCellThreads / 64];
+    const int set = blockIdx.x, tid = threadIdx.x;
+    const CellGrid cg = st.cg[set];
+    const int n = cg.n;
+    if (n == 0) {
+        if (tid < 3) cg.dim[tid] = 0;
+        return;
+    }
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int k = tid; k < n; k += kCellThreads) {
+        const double3 p = grid_point(cg, k);
+        lo[0] = fmin(lo[0], p.x); lo[1] = fmin(lo[1], p.y); lo[2] = fmin(lo[2], p.z);
+        hi[0] = fmax(hi[0], p.x); hi[1] = fmax(hi[1], p.y); hi[2] = fmax(hi[2], p.z);
+    }
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
+        }
+    const int lane = tid & 63, w = tid >> 6;
+    if (lane == 0)
+        for (int d = 0; d < 3; d++) { red[6 * w + d] = lo[d]; red[6 * w + 3 + d] = hi[d]; }
+    __syncthreads();
+    if (tid == 0) {
+        for (int i = 1; i < kCellThreads / 64; i++)
+            for (int d = 0; d < 3; d++) {
+                red[d] = fmin(red[d], red[6 * i + d]);
+                red[3 + d] = fmax(red[3 + d], red[6 * i + 3 + d]);
+            }
+        // a box that is not finite (a trajectory that blew up) takes one cell
+        double ext[3];
+        for (int d = 0; d < 3; d++) {
+            ext[d] = red[3 + d] - red[d];
+            if (!(ext[d] >= 0.0 && ext[d] < 1e12)) { ext[d] = 0.0; red[d] = 0.0; }
+        }
+        double edge = kReach;
+        int nd[3] = {1, 1, 1};
+        for (int it = 0; it < 200; it++) {
+            long long tot = 1;
+            for (int d = 0; d < 3; d++) {
+                nd[d] = (int)floor(ext[d] / edge) + 1;
+                tot *= nd[d];
+            }
+            if (tot <= kCellMax) break;
+            edge *= 1.25;
+        }
+        for (int d = 0; d < 3; d++) { s_hdr[d] = red[d]; s_dim[d] = nd[d]; cg.hdr[d] = red[d]; cg.dim[d] = nd[d]; }
+        s_hdr[3] = 1.0 / edge;
+        cg.hdr[3] = s_hdr[3];
+    }
+    __syncthreads();
+    const int nx = s_dim[0], ny = s_dim[1], nz = s_dim[2], nc = nx * ny * nz;
+    const double inv = s_hdr[3];
+    // counts, offsets and cursors: plain stores complete (the barrier waits
+    // for them) before the atomics of the next phase act on the same words,
+    // and the counts are read back through atomics, where they were formed
+    for (int c = tid; c <= nc; c += kCellThreads) cg.start[c] = 0;
+    __syncthreads();
+    for (int k = tid; k < n; k += kCellThreads) {
+        const int c = cell_of(grid_point(cg, k), s_hdr, inv, nx, ny, nz);
+        cg.cid[k] = c;
+        atomicAdd(cg.start + c, 1);
+    }
+    __syncthreads();
+    int carry = 0;
+    for (int base = 0; base < nc; base += kCellThreads) {
+        const int c = base + tid;
+        const int v = c < nc ? atomicAdd(cg.start + c, 0) : 0;
+        int tot;
+        const int ex = block_excl_scan<kCellThreads>(v, s_wsum, &tot);
+        if (c < nc) {
+            cg.start[c] = carry + ex;
+            cg.cursor[c] = carry + ex;
+        }
+        carry += tot;
+    }
+    if (tid == 0) cg.start[nc] = carry;
+    __syncthreads();
+    for (int k = tid; k < n; k += kCellThreads) {
+        const int slot = atomicAdd(cg.cursor + cg.cid[k], 1);
+        cg.members[slot] = k;
+    }
+}
+
+// Members of every cell in increasing index order, one wavefront per cell
+// (rank of each member = number of smaller members); a cell of more than
+// 256 members is ordered by one lane (insertion sort).
+__global__ void __launch_bounds__(256) k_cell_sort(Stat s, Sites st) {
+    const int lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * 4 + (threadIdx.x >> 6), tw = gridDim.x * 4;
+    for (int set = 0; set < NSETS; set++) {
+        if (set_size(s, set) == 0) continue;
+        const CellGrid cg = st.cg[set];
+        const int nc = cg.dim[0] * cg.dim[1] * cg.dim[2];
+        for (int c = gw; c < nc; c += tw) {
+            const int b = cg.start[c], m = cg.start[c + 1] - b;
+            if (m <= 1) continue;
+            int *mem = cg.members + b;
+            if (m <= 256) {
+                int x[4], r[4] = {0, 0, 0, 0};
+#pragma unroll
+                for (int i = 0; i < 4; i++) x[i] = (lane + 64 * i < m) ? mem[lane + 64 * i] : 0;
+                for (int j = 0; j < m; j++) {
+                    const int y = mem[j];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) r[i] += (y < x[i]) ? 1 : 0;
+                }
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if (lane + 64 * i < m) mem[r[i]] = x[i];
+            } else if (lane == 0) {
+                for (int i = 1; i < m; i++) {
+                    const int v = mem[i];
+                    int j = i - 1;
+                    while (j >= 0 && mem[j] > v) {
+                        mem[j + 1] = mem[j];
+                        j--;
+                    }
+                    mem[j + 1] = v;
+                }
+            }
+        }
+    }
+}
+
+// The partners within reach of the sphere (p, rp) through a cell grid: the
+// runs of cells along x that meet the box [p - rp - reach, p + rp + reach],
+// rows in (z, y) order, each run a contiguous range of members, dealt out
+// in that order, member f of the concatenation going to the flat thread
+// index f (mod stride) -- start is this thread's flat index.  Partners left
+// out are beyond the 7 nm cut-off of every term that uses this, where the
+// terms are exactly 0.  Contains barriers: every thread of the workgroup
+// calls it with the same sphere.
+template <int NT, typename Fn>
+__device__ __forceinline__ void for_near_cells(const CellGrid &cg, double3 p, double rp, int start, int stride,
+                                               int *off, int *wsum, Fn &&fn) {
+    __shared__ int s_beg[NT];
+    const int nx = cg.dim[0], ny = cg.dim[1], nz = cg.dim[2];
+    if (nx == 0) return;
+    const double inv = cg.hdr[3], reach = rp + kReach;
+    const double pc[3] = {p.x, p.y, p.z};
+    const int nd[3] = {nx, ny, nz};
+    int lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        lo[d] = max((int)floor((pc[d] - reach - cg.hdr[d]) * inv), 0);
+        hi[d] = min((int)floor((pc[d] + reach - cg.hdr[d]) * inv), nd[d] - 1);
+    }
+    if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) return;
+    const int nyr = hi[1] - lo[1] + 1, nruns = nyr * (hi[2] - lo[2] + 1);
+    for (int rb = 0; rb < nruns; rb += NT) {
+        const int r = rb + threadIdx.x;
+        int beg = 0, len = 0;
+        if (r < nruns) {
+            const int c0 = ((lo[2] + r / nyr) * ny + lo[1] + r % nyr) * nx + lo[0];
+            beg = cg.start[c0];
+            len = cg.start[c0 + hi[0] - lo[0] + 1] - beg;
+        }
+        int tot;
+        const int ex = block_excl_scan<NT>(len, wsum, &tot);
+        off[threadIdx.x] = ex;
+        s_beg[threadIdx.x] = beg;
+        __syncthreads();
+        const int nr = min(NT, nruns - rb);
+        for (int f = start; f < tot; f += stride) {
+            int a = 0, z = nr - 1;  // last run with off <= f
+            while (a < z) {
+                const int mid = (a + z + 1) >> 1;
+                if (off[mid] <= f) a = mid;
+                else z = mid - 1;
+            }
+            fn(cg.members[s_beg[a] + f - off[a]]);
+        }
+        __syncthreads();
+    }
+}
+
+// The partners of set `set` within reach of (p, rp): through the cell
+// grid, or through the chunk spheres ch.
+template <int NT, typename Fn>
+__device__ __forceinline__ void for_near_set(const Sites &st, int set, const double4 *ch, int n, double3 p, double rp,
+                                             int start, int stride, int *list, int *wsum, Fn &&fn) {
+    if (st.use_cells) for_near_cells<NT>(st.cg[set], p, rp, start, stride, list, wsum, fn);
+    else for_near<NT>(ch, n, p, rp, start, stride, list, wsum, fn);
+}
+
 // Sphere around a set of points (centroid, largest distance from it).
 template <int N>
 __device__ __forceinline__ void points_sphere(const double3 (&x)[N], int n, double3 &c, double &r) {
@@ -402,18 +633,18 @@ __global__ void __launch_bounds__(kLinkerThreads) k_linker(Stat s, Geo g, Sites 
         }
     }
     if (terms & TERM_TAIL)  // tail-linker, energy owned by tails
-        for_near<kLinkerThreads>(st.ch_t, s.n_tail, rj, 0.0, tid, nth, s_list, s_wsum, [&](int t) {
+        for_near_set<kLinkerThreads>(st, SET_TAIL, st.ch_t, s.n_tail, rj, 0.0, tid, nth, s_list, s_wsum, [&](int t) {
             dhev_recv(s.ec, s.t_q[t] * s.q_l, s.c4pe, s.debye, s.k_ex, s.sig_tl, rj, ld3(g.r_t + 3 * t), F, Ed);
         });
     if (terms & TERM_LH) {
         // LH-DNA, energy owned by LH beads
-        for_near<kLinkerThreads>(st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int i) {
+        for_near_set<kLinkerThreads>(st, SET_LH, st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int i) {
             int idx = i % s.nlh;
             idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
             dhev_recv(s.ec, s.LH_q[idx] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[idx], rj, ld3(g.r_lh + 3 * i), F, Ed);
         });
         // LHG-DNA, energy on the core side
-        for_near<kLinkerThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
+        for_near_set<kLinkerThreads>(st, SET_HEAD, st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
             int m = x % s.n_lh_g;
             dhev_recv(s.ec, s.LH_q[m] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[m], rj, ld3(st.G + 3 * x), F, Ed);
         });
@@ -466,24 +697,24 @@ __global__ void __launch_bounds__(kTailThreads) k_tail(Stat s, Geo g, Sites st, 
         }
         // tail-tail DH (1863-1891), skipping same-group neighbours
         const double qj = s.t_q[j];
-        for_near<kTailThreads>(st.ch_t, nt, rj, 0.0, tid, nth, s_list, s_wsum, [&](int k) {
+        for_near_set<kTailThreads>(st, SET_TAIL, st.ch_t, nt, rj, 0.0, tid, nth, s_list, s_wsum, [&](int k) {
             if (k == j) return;
             if (!(tg[k] != tg[j] || abs(k - j) > 2)) return;
             dh_recv(s.ec, qj * s.t_q[k], s.c4pe, s.debye, rj, ld3(g.r_t + 3 * k), F, (k > j) ? E : Ed);
         });
         // tail-linker (1898-1917)
-        for_near<kTailThreads>(st.ch_l, s.n_lnk, rj, 0.0, tid, nth, s_list, s_wsum, [&](int pi) {
+        for_near_set<kTailThreads>(st, SET_LINKER, st.ch_l, s.n_lnk, rj, 0.0, tid, nth, s_list, s_wsum, [&](int pi) {
             dhev_recv(s.ec, qj * s.q_l, s.c4pe, s.debye, s.k_ex, s.sig_tl, rj, ld3(g.r + 3 * s.lnk[pi]), F, E);
         });
     }
     if ((terms & TERM_LH) && s.t_fix[j] == 0) {
         // LH-tail and LHG-tail, tail side; anchored tails pass these to a core (k_core)
-        for_near<kTailThreads>(st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int i) {
+        for_near_set<kTailThreads>(st, SET_LH, st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int i) {
             int idx = i % s.nlh;
             idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
             dhev_recv(s.ec, s.LH_q[idx] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rj, ld3(g.r_lh + 3 * i), F, Ed);
         });
-        for_near<kTailThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
+        for_near_set<kTailThreads>(st, SET_HEAD, st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
             int m = x % s.n_lh_g;
             dhev_recv(s.ec, s.LH_q[m] * s.t_q[j], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], rj, ld3(st.G + 3 * x), F, Ed);
         });
@@ -543,7 +774,7 @@ __global__ void __launch_bounds__(kLHThreads) k_lh(Stat s, Geo g, Sites st, Frc 
             }
         }
         // LH-LH (2190-2244)
-        for_near<kLHThreads>(st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int k) {
+        for_near_set<kLHThreads>(st, SET_LH, st.ch_lh, s.n_LH, rj, 0.0, tid, nth, s_list, s_wsum, [&](int k) {
             if (k == j) return;
             int i2 = tpl(k);
             double sg = (s.LH_hh[idx] + s.LH_hh[i2]) / 2;
@@ -555,15 +786,15 @@ __global__ void __launch_bounds__(kLHThreads) k_lh(Stat s, Geo g, Sites st, Frc 
             }
         });
         // LH-DNA (2249-2279)
-        for_near<kLHThreads>(st.ch_l, s.n_lnk, rj, 0.0, tid, nth, s_list, s_wsum, [&](int pi) {
+        for_near_set<kLHThreads>(st, SET_LINKER, st.ch_l, s.n_lnk, rj, 0.0, tid, nth, s_list, s_wsum, [&](int pi) {
             dhev_recv(s.ec, s.LH_q[idx] * s.q_l, s.c4pe, s.debye, s.k_ex, s.LH_hl[idx], rj, ld3(g.r + 3 * s.lnk[pi]), F, E);
         });
         // LH-tail (2343-2398)
-        for_near<kLHThreads>(st.ch_t, s.n_tail, rj, 0.0, tid, nth, s_list, s_wsum, [&](int t) {
+        for_near_set<kLHThreads>(st, SET_TAIL, st.ch_t, s.n_tail, rj, 0.0, tid, nth, s_list, s_wsum, [&](int t) {
             dhev_recv(s.ec, s.LH_q[idx] * s.t_q[t], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rj, ld3(g.r_t + 3 * t), F, E);
         });
         // LHG of other cores acting on this bead (2590-2644), energy on the core side
-        for_near<kLHThreads>(st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
+        for_near_set<kLHThreads>(st, SET_HEAD, st.ch_g, s.n_lhc * s.n_lh_g, rj, 0.0, tid, nth, s_list, s_wsum, [&](int x) {
             int k = x / s.n_lh_g, m = x % s.n_lh_g;
             if (k == mine) return;
             dhev_recv(s.ec, s.LH_q[m] * s.LH_q[idx], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[idx]) / 2, rj,
@@ -714,14 +945,14 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
         double hr;
         points_sphere(Gm, ng, hc, hr);
         // LHG-tail (2403-2476)
-        for_near<kCoreThreads>(st.ch_t, s.n_tail, hc, hr, ft, FT, s_list, s_wsum, [&](int t) {
+        for_near_set<kCoreThreads>(st, SET_TAIL, st.ch_t, s.n_tail, hc, hr, ft, FT, s_list, s_wsum, [&](int t) {
             double3 rt = ld3(g.r_t + 3 * t);
 #pragma unroll
             for (int m = 0; m < kMaxGSites; m++) if (m < ng)
                 dhev_recv(s.ec, s.LH_q[m] * s.t_q[t], s.c4pe, s.debye, s.k_ex, s.LH_ht[m], Gm[m], rt, Fm[m], E);
         });
         // LHG-DNA (2481-2522)
-        for_near<kCoreThreads>(st.ch_l, s.n_lnk, hc, hr, ft, FT, s_list, s_wsum, [&](int pi) {
+        for_near_set<kCoreThreads>(st, SET_LINKER, st.ch_l, s.n_lnk, hc, hr, ft, FT, s_list, s_wsum, [&](int pi) {
             double3 rl = ld3(g.r + 3 * s.lnk[pi]);
 #pragma unroll
             for (int m = 0; m < kMaxGSites; m++) if (m < ng)
@@ -753,7 +984,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
             __syncthreads();
         }
         // LHG-free LH beads of other cores (2590-2644)
-        for_near<kCoreThreads>(st.ch_lh, s.n_LH, hc, hr, ft, FT, s_list, s_wsum, [&](int jl) {
+        for_near_set<kCoreThreads>(st, SET_LH, st.ch_lh, s.n_LH, hc, hr, ft, FT, s_list, s_wsum, [&](int jl) {
             if (jl / s.nlh == lho) return;
             int i2 = jl % s.nlh;
             i2 = (i2 < s.n_lh_n) ? i2 : i2 + s.n_lh_g;
@@ -763,7 +994,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
                 dhev_recv(s.ec, s.LH_q[m] * s.LH_q[i2], s.c4pe, s.debye, s.k_ex, (s.LH_hh[m] + s.LH_hh[i2]) / 2, Gm[m], rl, Fm[m], E);
         });
         // LHG-LHG with the other LH-bound cores (2649-2719)
-        for_near<kCoreThreads>(st.ch_g, s.n_lhc * ng, hc, hr, ft, FT, s_list, s_wsum, [&](int x) {
+        for_near_set<kCoreThreads>(st, SET_HEAD, st.ch_g, s.n_lhc * ng, hc, hr, ft, FT, s_list, s_wsum, [&](int x) {
             int k = x / ng, m1 = x % ng;
             if (k == lho) return;
             double3 G2 = ld3(st.G + 3 * x);
@@ -812,7 +1043,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
             double3 tc;
             double tr;
             points_sphere(rt, ne, tc, tr);
-            for_near<kCoreThreads>(st.ch_lh, s.n_LH, tc, tr, ft, FT, s_list, s_wsum, [&](int i) {
+            for_near_set<kCoreThreads>(st, SET_LH, st.ch_lh, s.n_LH, tc, tr, ft, FT, s_list, s_wsum, [&](int i) {
                 int idx = i % s.nlh;
                 idx = (idx < s.n_lh_n) ? idx : idx + s.n_lh_g;
                 const double3 rl = ld3(g.r_lh + 3 * i);
@@ -820,7 +1051,7 @@ __global__ void __launch_bounds__(kCoreThreads) k_core(Stat s, Geo g, Sites st, 
                 for (int e = 0; e < kMaxRT; e++)
                     if (e < ne) dhev_recv(s.ec, s.LH_q[idx] * tq[e], s.c4pe, s.debye, s.k_ex, s.LH_ht[idx], rt[e], rl, Ft[e], Ed);
             });
-            for_near<kCoreThreads>(st.ch_g, s.n_lhc * s.n_lh_g, tc, tr, ft, FT, s_list, s_wsum, [&](int x) {
+            for_near_set<kCoreThreads>(st, SET_HEAD, st.ch_g, s.n_lhc * s.n_lh_g, tc, tr, ft, FT, s_list, s_wsum, [&](int x) {
                 const int m = x % s.n_lh_g;
                 const double3 Gx = ld3(st.G + 3 * x);
 #pragma unroll
@@ -1068,6 +1299,25 @@ static inline int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
 void launch_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, hipStream_t q) {
     long long tot = (long long)s.n_c * s.Nq + (long long)s.n_lhc * s.n_lh_g;
     k_sites<<<nblk(std::max(tot, 1LL), 256), 256, 0, q>>>(s, g, st, cc_count);
+    if (st.use_cells) {
+        // the grids are built from this pass's positions
+        Sites sp = st;
+        sp.cg[SET_TAIL].pos = g.r_t;
+        sp.cg[SET_TAIL].idx = nullptr;
+        sp.cg[SET_TAIL].n = s.n_tail;
+        sp.cg[SET_LH].pos = g.r_lh;
+        sp.cg[SET_LH].idx = nullptr;
+        sp.cg[SET_LH].n = s.n_LH;
+        sp.cg[SET_HEAD].pos = st.G;
+        sp.cg[SET_HEAD].idx = nullptr;
+        sp.cg[SET_HEAD].n = s.n_lhc * s.n_lh_g;
+        sp.cg[SET_LINKER].pos = g.r;
+        sp.cg[SET_LINKER].idx = s.lnk;
+        sp.cg[SET_LINKER].n = s.n_lnk;
+        k_cells<<<NSETS, kCellThreads, 0, q>>>(sp);
+        k_cell_sort<<<512, 256, 0, q>>>(s, st);
+        return;
+    }
     const int nch = nchunks(s.n_tail) + nchunks(s.n_LH) + nchunks(s.n_lhc * s.n_lh_g) + nchunks(s.n_lnk);
     if (nch > 0) k_chunks<<<nblk(nch, 4), 256, 0, q>>>(s, g, st);
 }

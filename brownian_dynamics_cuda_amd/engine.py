@@ -83,6 +83,7 @@ API = {
     "bd_noise_displacement": ([C.c_void_p, _dp, _dp], C.c_int),
     "bd_set_chol_mode": ([C.c_void_p, C.c_int], C.c_int),
     "bd_set_chol_schedule": ([C.c_void_p, C.c_int], C.c_int),
+    "bd_set_cell_list": ([C.c_void_p, C.c_int], C.c_int),
     "bd_cc_forward": ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
                       + [C.c_double] * 4 + [C.c_void_p] * 3, C.c_int),
     "bd_cc_backward": ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -347,6 +348,12 @@ class BDEngine:
         """Super-panel width of the factor in 64-column block columns: 0
         chooses by size, -1 one block column per launch, k >= 2 always k."""
         _check(self.lib.bd_set_chol_schedule(self.h, int(panel_tiles)), "bd_set_chol_schedule")
+
+    def set_cell_list(self, enable: bool):
+        """Partner search of the 7 nm terms: cell grids (True; the default
+        from 20000 tail and LH beads up) or bounding spheres of 16
+        consecutive partners."""
+        _check(self.lib.bd_set_cell_list(self.h, 1 if enable else 0), "bd_set_cell_list")
 
     def chol_schedule(self) -> int:
         """Super-panel width in use (0: one block column per launch)."""

@@ -154,6 +154,16 @@ int bd_get_tensor(bd_context *ctx, double *D);
 int bd_noise_displacement(bd_context *ctx, const double *p, double *rd);
 int bd_set_chol_mode(bd_context *ctx, int chol_mode);
 
+/* Partner search of the short-range terms (tail, linker-histone, head-site
+ * and linker partners within the 7 nm cut-off of func.cpp:278-371, which
+ * the reference evaluates over all pairs, func.cpp:1863-2022, 2109-2777):
+ * enable = 1 through cell grids of edge >= 7 nm rebuilt every force pass,
+ * 0 through bounding spheres of 16 consecutive partners.  Both evaluate
+ * exactly the reference's nonzero pairs; only the summation order differs.
+ * bd_create chooses cells for fibers of 20000 tail and LH beads or more;
+ * the environment variable BD_CELLS overrides that choice. */
+int bd_set_cell_list(bd_context *ctx, int enable);
+
 /* Schedule of the blocked factor (no counterpart in the reference, which
  * calls cusolverDnDpotrf, func_cuda.cu:3526-3529).  panel_tiles = 0 (the
  * default) chooses: super-panels of 8 block columns of 64 from 64 block

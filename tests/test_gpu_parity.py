@@ -421,3 +421,29 @@ def test_frequency_rp_against_oracle_reuse(hi):
             got = eng.get_state()
             for k in ("r", "a", "b", "c", "r_t", "r_lh"):
                 assert rel_err(getattr(got, k), fs.view(k)) <= 1e-9, (s, k)
+
+
+@pytest.mark.parametrize("kind", ["small", "cfg2"])
+def test_cell_list_matches_chunk_search(kind):
+    """The 7 nm cell grids (bd_set_cell_list(1), the default) and the chunk
+    spheres (0) evaluate the same pair set: forces within summation-order
+    rounding of each other and of the oracle; one graph step each."""
+    fib, st = deck(kind) if kind != "cfg2" else Fb.make_config(2)
+    orc = O.Oracle(fib)
+    fs = orc.prepare(st)
+    rr, p = noise(fib)
+    got = {}
+    for cells in (True, False):
+        with E.BDEngine(fib) as eng:
+            eng.set_cell_list(cells)
+            sync_engine(eng, fs)
+            f = eng.forces(E.TERM_ALL)
+            e = eng.step(rr, p)
+            got[cells] = (f, e, eng.get_state())
+    ref = orc.forces(fs, ("all",))
+    for k in ("F", "T", "tF", "lhF"):
+        assert rel_err(got[True][0][k], got[False][0][k]) <= 1e-13, k
+        assert rel_err(got[True][0][k], ref[k]) <= TOL, k
+    assert got[True][1] == pytest.approx(got[False][1], rel=1e-13)
+    for k in ("r", "a", "b", "c", "r_t", "r_lh"):
+        assert rel_err(getattr(got[True][2], k), getattr(got[False][2], k)) <= 1e-12, k
