@@ -234,7 +234,8 @@ def main():
 
     # mixed-precision factors: same state and noise, one step each way;
     # displacement error against FP64, factor error ||L L^T - D|| / ||D||
-    # (through random probes L (L^T v) - D v above 4096 rows), steps/s
+    # (through random probes L (L^T v) - D v above 4096 rows), the covariance
+    # error of the FP64-refined noise, steps/s (refinement on, the default)
     mixed = None
     if not args.no_mixed:
         mixed = {}
@@ -265,6 +266,19 @@ def main():
                     v = g.standard_normal(D.shape[0])
                     Dv = D @ v
                     ferr = max(ferr, float(np.linalg.norm(L @ (L.T @ v) - Dv) / np.linalg.norm(Dv)))
+            # covariance of the refined noise, B = L (I + X / 2) with
+            # X = inv(L) D inv(L)^T - I (what bd_set_refine applies):
+            # ||B B^T v - D v|| / ||D v|| on random probes
+            from scipy.linalg import solve_triangular
+            g = np.random.default_rng(1)
+            cov = 0.0
+            for _ in range(3):
+                v = g.standard_normal(D.shape[0])
+                y = L.T @ v
+                bt = 0.5 * (y + solve_triangular(L, D @ solve_triangular(L.T, y, lower=False), lower=True))
+                bbt = 0.5 * (L @ bt + D @ solve_triangular(L.T, bt, lower=False))
+                Dv = D @ v
+                cov = max(cov, float(np.linalg.norm(bbt - Dv) / np.linalg.norm(Dv)))
             del D, L
             engm.set_state(st, rebuild=True)
             d_e2 = E.DeviceBuffer(8 * (W + K))
@@ -277,10 +291,11 @@ def main():
             engm.synchronize()
             dm = time.perf_counter() - t1
             mixed[name] = dict(steps_per_s=K / dm, ms_per_step=1e3 * dm / K,
-                               rel_displacement_err_vs_fp64=rel, factor_rel_err=ferr)
+                               rel_displacement_err_vs_fp64=rel, factor_rel_err=ferr,
+                               refined_covariance_rel_err=cov)
             engm.close()
             print(f"[bench] {name}: {K / dm:.2f} steps/s, displacement err {rel:.2e}, "
-                  f"factor err {ferr:.2e}", file=sys.stderr, flush=True)
+                  f"factor err {ferr:.2e}, refined covariance err {cov:.2e}", file=sys.stderr, flush=True)
 
     # CPU baseline: the reference's own func.cpp composed into one step
     cpu = None

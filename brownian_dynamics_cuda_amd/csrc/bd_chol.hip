@@ -999,12 +999,15 @@ __global__ void __launch_bounds__(256) k_diag(double *A, int ld, int d, double *
 
 // rd = s2dt * L * p (rd_cal, func_cuda.cu:634-647): one wavefront per row,
 // lane k % 64 over the row; rows outside the hydrodynamic set use their
-// self-mobility: s2dt * sqrt(a1 / rad) * p.  L is read where the factor
+// self-mobility: s2dt * sqrt(a1 / rad) * p.  With du = D inv(L~)^T p (the
+// FP64 refinement of a reduced-precision factor) the HI rows take
+// s2dt (L~ p + du) / 2 instead.  L is read where the factor
 // leaves it: the diagonal tile of the row in place, the off-diagonal tile
 // (ib, t) parked in the upper block (t, ib) (a row of the parked tile is
 // contiguous, so every read is one 512-byte segment).
 __global__ void __launch_bounds__(256) k_noise(const double *A, int ld, int m, int nD3, const double *p,
-                                               const double *rad_all, double s2dt, double a1, double *rd) {
+                                               const double *du, const double *rad_all, double s2dt, double a1,
+                                               double *rd) {
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= nD3) return;
@@ -1031,13 +1034,139 @@ __global__ void __launch_bounds__(256) k_noise(const double *A, int ld, int m, i
     for (; t < ib; t++) v += a[(size_t)t * TB * ld] * q[t * TB];
     if (lane <= r) v += A[(size_t)row * ld + ib * TB + lane] * p[ib * TB + lane];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    if (lane == 0) rd[row] = s2dt * v;
+    if (lane == 0) rd[row] = s2dt * (du ? 0.5 * (v + du[row]) : v);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 refinement of B xi for a factor whose trailing updates ran in FP32 or
+// BF16.  With L~ L~^T = D + E and M = inv(L~) D inv(L~)^T = I + X (X small
+// and symmetric), B = L~ M^(1/2) satisfies B B^T = D exactly, and to first
+// order B xi = L~ (xi + X xi / 2).  Since L~ X xi = D u - L~ xi for
+// u = inv(L~)^T xi (L~^T u = xi), that is
+//   B xi = (L~ xi + D u) / 2,   B B^T - D = L~ X^2 L~^T / 4 + O(X^3):
+// one FP64 triangular solve with the factor (k_bsolve), one matrix-free
+// product D u from the bead positions D was assembled from (k_dmul, the RPY
+// blocks of k_rpy), and the mean taken in k_noise beside L~ xi.
+// ---------------------------------------------------------------------------
+__global__ void k_hi_snapshot(Stat s, Geo g, int nhi, double *xs) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nhi) return;
+    const double3 r = hi_pos(s, g, b);
+    xs[3 * b] = r.x;
+    xs[3 * b + 1] = r.y;
+    xs[3 * b + 2] = r.z;
+}
+
+__global__ void k_refine_init(const double *p, int m, int mpad, double *pw) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < mpad) pw[r] = r < m ? p[r] : 0.0;
+}
+
+// Column part of a 64x64 tile T (global, leading dimension ldt) for this
+// thread: column e = t % 64, rows 16q .. 16q+15, q = t / 64 (a wavefront
+// reads one row per load), loaded into registers ahead of use.
+__device__ __forceinline__ void col_load(const double *T, size_t ldt, double (&v)[16]) {
+    const int e = threadIdx.x & 63, q = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 16; j++) v[j] = T[(16 * q + j) * ldt + e];
+}
+// part[q][e] = sum_j v[j] x[16q + j], then y[e] = sum_q part[q][e] in a
+// fixed order for e < 64 (threads 0..63), with barriers around.
+__device__ __forceinline__ void col_reduce(const double (&v)[16], const double *x, double *part, double *y,
+                                           const double *sub) {
+    const int t = threadIdx.x, e = t & 63, q = t >> 6;
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) a += v[j] * x[16 * q + j];
+    part[q * TB + e] = a;
+    __syncthreads();
+    if (t < TB) {
+        const double sum = ((part[t] + part[TB + t]) + part[2 * TB + t]) + part[3 * TB + t];
+        y[t] = sub ? sub[t] - sum : sum;
+    }
+    __syncthreads();
+}
+
+// Blocks b0 = 2g and b1 = 2g + 1 (if present) of u = inv(L~)^T xi; the
+// blocks after them were solved by earlier launches, so pw there is final:
+//   u_b1 = inv(L~_b1b1)^T pw_b1,  u_b0 = inv(L~_b0b0)^T (pw_b0 - L~_b1b0^T u_b1)
+// in every workgroup (all tiles loaded at the start); workgroup 0 stores
+// u_b0 and u_b1, workgroup k >= 1 subtracts L~_b0,i^T u_b0 + L~_b1,i^T u_b1
+// from pw_i for tile row i = k - 1 < b0 (L~_b,i parked in block (i, b)).
+__global__ void __launch_bounds__(256) k_bsolve(const double *A, int ld, const double *Linv, int nt, int g,
+                                                double *pw, double *u) {
+    __shared__ double x[2 * TB], u0[TB], u1[TB], part[4 * TB];
+    const int t = threadIdx.x, b0 = 2 * g, two = b0 + 1 < nt, i = blockIdx.x - 1;
+    double la[16], lc[16], lb[16], r0[16], r1[16];
+    if (two) {
+        col_load(Linv + (size_t)(b0 + 1) * TB * TB, TB, la);
+        col_load(A + (size_t)b0 * TB * ld + (b0 + 1) * TB, ld, lc);
+    }
+    col_load(Linv + (size_t)b0 * TB * TB, TB, lb);
+    if (i >= 0) {
+        col_load(A + (size_t)i * TB * ld + b0 * TB, ld, r0);
+        if (two) col_load(A + (size_t)i * TB * ld + (b0 + 1) * TB, ld, r1);
+    }
+    if (t < (two ? 2 : 1) * TB) x[t] = pw[b0 * TB + t];
+    __syncthreads();
+    if (two) {
+        col_reduce(la, x + TB, part, u1, nullptr);
+        col_reduce(lc, u1, part, x, x);  // pw_b0 - L~_b1b0^T u_b1, in place
+    }
+    col_reduce(lb, x, part, u0, nullptr);
+    if (i < 0) {
+        if (t < TB) u[b0 * TB + t] = u0[t];
+        else if (two && t < 2 * TB) u[b0 * TB + t] = u1[t - TB];
+        return;
+    }
+    const int e = t & 63, q = t >> 6;
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) a += r0[j] * u0[16 * q + j];
+    if (two)
+#pragma unroll
+        for (int j = 0; j < 16; j++) a += r1[j] * u1[16 * q + j];
+    part[q * TB + e] = a;
+    __syncthreads();
+    if (t < TB) pw[i * TB + t] -= ((part[t] + part[TB + t]) + part[2 * TB + t]) + part[3 * TB + t];
+}
+
+// w = D u for the hydrodynamic set (m = 3 nhi rows): one wavefront per bead
+// i, lanes over the beads j, the 3x3 blocks of k_rpy (a1/rad_i on the
+// diagonal), lane sums combined in a fixed order.
+__global__ void __launch_bounds__(256) k_dmul(const double *xs, const double *rad_all, int nhi, double a1, double a2,
+                                              const double *u, double *w) {
+    const int lane = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nhi) return;
+    const double3 ri = make_double3(xs[3 * i], xs[3 * i + 1], xs[3 * i + 2]);
+    const double radi = rad_all[i];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int j = lane; j < nhi; j += 64) {
+        const double uj[3] = {u[3 * j], u[3 * j + 1], u[3 * j + 2]};
+        if (j == i) {
+            const double d = a1 / radi;
+#pragma unroll
+            for (int k = 0; k < 3; k++) acc[k] += d * uj[k];
+        } else {
+            double M[3][3];
+            rpy_block(ri, make_double3(xs[3 * j], xs[3 * j + 1], xs[3 * j + 2]), radi, rad_all[j], a1, a2, M);
+#pragma unroll
+            for (int k = 0; k < 3; k++) acc[k] += (M[k][0] * uj[0] + M[k][1] * uj[1]) + M[k][2] * uj[2];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_down(acc[k], o, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 3; k++) w[3 * i + k] = acc[k];
 }
 
 // ---------------------------------------------------------------------------
 void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, const CholWork &c, hipStream_t q,
                 bool factor_first) {
     const int nt = c.mpad / TB;
+    if (c.xs && c.m > 0) k_hi_snapshot<<<nblk(c.m / 3, 256), 256, 0, q>>>(s, g, c.m / 3, c.xs);
     if (factor_first)
         k_rpy<true><<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2, c.neg, c.Linv);
     else
@@ -1148,9 +1277,19 @@ void launch_unpark(const CholWork &c, double *L, int ldl, hipStream_t q) {
     k_unpark<<<grid, 256, 0, q>>>(c.A, c.ld, c.m, L, ldl);
 }
 
+bool refine_active(const CholWork &c) { return c.mode != 0 && c.refine && c.m > 0 && c.xs; }
+
+void launch_refine(const Stat &s, const CholWork &c, const double *p, const double *rad_all, hipStream_t q) {
+    const int nt = c.mpad / TB, nhi = c.m / 3, ng = nblk(nt, 2);
+    double *pw = c.rv, *u = c.rv + c.ld;
+    k_refine_init<<<nblk(c.mpad, 256), 256, 0, q>>>(p, c.m, c.mpad, pw);
+    for (int g = ng - 1; g >= 0; g--) k_bsolve<<<1 + 2 * g, 256, 0, q>>>(c.A, c.ld, c.Linv, nt, g, pw, u);
+    k_dmul<<<nblk(nhi, 4), 256, 0, q>>>(c.xs, rad_all, nhi, s.a1, s.a2, u, c.pr);
+}
+
 void launch_noise(const Stat &s, int n_hi, int n_D3, const CholWork &c, const double *p, const double *rad_all,
-                  double *rd, hipStream_t q) {
-    k_noise<<<nblk(n_D3, 4), 256, 0, q>>>(c.A, c.ld, c.m, n_D3, p, rad_all, s.s2dt, s.a1, rd);
+                  double *rd, hipStream_t q, const double *du) {
+    k_noise<<<nblk(n_D3, 4), 256, 0, q>>>(c.A, c.ld, c.m, n_D3, p, du, rad_all, s.s2dt, s.a1, rd);
     (void)n_hi;
 }
 

@@ -56,7 +56,7 @@ int fail(int code, const char *fmt, ...) {
 const char *kKernelNames[K_COUNT] = {
     "sites", "mech", "linker", "tail", "lh", "core", "cc_list", "cc_work", "rpy",
     "chol_step", "noise", "frames", "betas", "rotate", "translate", "average",
-    "commit", "energy_reduce", "cc_gather", "chol_trail", "chol_diag"};
+    "commit", "energy_reduce", "cc_gather", "chol_trail", "chol_diag", "refine"};
 
 }  // namespace
 
@@ -222,7 +222,12 @@ void enqueue_step(bd_context *c, bool refactor) {
     const int gate_kb = c->gate_kb >= 0 ? c->gate_kb : (nt - 1) * 55 / 100;
     const bool gated = refactor && qf != q;
     if (refactor) enqueue_factor(c, c->cur, qf, gated ? c->ev_gate : nullptr, gate_kb);
-    M(K_NOISE, 0); launch_noise(s, c->n_hi, c->n_D3, c->chol, c->p, c->rad_all, c->rd, qf); M(K_NOISE, 1);
+    const double *du = nullptr;
+    if (refine_active(c->chol)) {
+        M(K_REFINE, 0); launch_refine(s, c->chol, c->p, c->rad_all, qf); M(K_REFINE, 1);
+        du = c->chol.pr;
+    }
+    M(K_NOISE, 0); launch_noise(s, c->n_hi, c->n_D3, c->chol, c->p, c->rad_all, c->rd, qf, du); M(K_NOISE, 1);
     if (gated) (void)hipStreamWaitEvent(q, c->ev_gate, 0);
     // LH bending angles at the current state (3497)
     M(K_BETAS, 0); launch_betas(s, c->cur, 0, 1, q); M(K_BETAS, 1);
@@ -542,6 +547,11 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     r |= c->alloc(&c->chol.A, (size_t)c->chol.ld * c->chol.ld);
     r |= c->alloc(&c->chol.Linv, (size_t)(c->chol.ld / 64) * 64 * 64);
     r |= c->alloc(&c->chol.neg, 1);
+    c->chol.refine = 1;
+    if (const char *ev = getenv("BD_REFINE")) c->chol.refine = atoi(ev) != 0;
+    r |= c->alloc(&c->chol.xs, (size_t)c->chol.ld);
+    r |= c->alloc(&c->chol.rv, 2 * (size_t)c->chol.ld);
+    r |= c->alloc(&c->chol.pr, (size_t)c->chol.ld);
     r |= c->alloc(&c->rd, 3 * nD_cap);
     r |= c->alloc(&c->p, 3 * nD_cap);
     r |= c->alloc(&c->rr, 3 * n);
@@ -883,7 +893,12 @@ int bd_noise_displacement(bd_context *c, const double *p, double *rd) {
     if (check(c) || !p || !rd) return fail(BD_E_ARG, "null argument");
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(c->p, p, sizeof(double) * c->n_D3, hipMemcpyHostToDevice, c->stream));
-    launch_noise(c->st, c->n_hi, c->n_D3, c->chol, c->p, c->rad_all, c->rd, c->stream);
+    const double *du = nullptr;
+    if (refine_active(c->chol)) {
+        launch_refine(c->st, c->chol, c->p, c->rad_all, c->stream);
+        du = c->chol.pr;
+    }
+    launch_noise(c->st, c->n_hi, c->n_D3, c->chol, c->p, c->rad_all, c->rd, c->stream, du);
     int e = sync_check(c);
     if (e) return e;
     return d2h(rd, c->rd, c->n_D3);
@@ -900,6 +915,17 @@ int bd_set_chol_mode(bd_context *c, int mode) {
         c->rp_count = 0;
     }
     c->chol.mode = mode;
+    return BD_OK;
+}
+
+int bd_set_refine(bd_context *c, int enable) {
+    if (check(c)) return BD_E_ARG;
+    const int v = enable != 0;
+    if (v != c->chol.refine) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        drop_graphs(c);
+    }
+    c->chol.refine = v;
     return BD_OK;
 }
 

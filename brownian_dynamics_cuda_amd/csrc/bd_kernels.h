@@ -31,7 +31,7 @@ void launch_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, h
 // (phase 0 before, 1 after); ids are the K_* values below.
 enum KernelId { K_SITES, K_MECH, K_LINKER, K_TAIL, K_LH, K_CORE, K_CCLIST, K_CCWORK,
                 K_RPY, K_CHOL_STEP, K_NOISE, K_FRAMES, K_BETAS, K_ROTATE, K_TRANSLATE,
-                K_AVG, K_COMMIT, K_EREDUCE, K_CCGATHER, K_CHOL_TRAIL, K_CHOL_DIAG, K_COUNT };
+                K_AVG, K_COMMIT, K_EREDUCE, K_CCGATHER, K_CHOL_TRAIL, K_CHOL_DIAG, K_REFINE, K_COUNT };
 typedef void (*MarkFn)(void *, int, int);
 // Extra streams of a force pass: after the charge sites, the tail, the
 // linker-histone (then, once mechanics is done, linker) and the core-core
@@ -79,6 +79,11 @@ struct CholWork {
     // super-panel width in block columns: 0 chooses (8 from 64 block
     // columns up, none below), -1 never, k >= 2 always k
     int panel_tiles;
+    // FP64 refinement of B xi in the FP32 / BF16 modes (1: on): xs, the
+    // hydrodynamic-set positions D was last assembled from (ld doubles);
+    // rv, two work vectors of ld doubles; pr, D inv(L~)^T p (ld)
+    int refine;
+    double *xs, *rv, *pr;
 };
 // super-panel width the factor of c uses (0: one block column at a time)
 int chol_panel_tiles(const CholWork &c);
@@ -99,6 +104,11 @@ void launch_diag(const CholWork &c, int d, bool clear, hipStream_t q);
 void launch_pack(const double *D, int ldd, const CholWork &c, hipStream_t q);
 void launch_unpark(const CholWork &c, double *L, int ldl, hipStream_t q);
 void launch_noise(const Stat &s, int n_hi, int n_D3, const CholWork &c, const double *p,
-                  const double *rad_all, double *rd, hipStream_t q);
+                  const double *rad_all, double *rd, hipStream_t q, const double *du = nullptr);
+// whether the noise of factor c takes the FP64 refinement, and the
+// refinement itself: c.pr = D inv(L~)^T p over the HI rows (the du of
+// launch_noise)
+bool refine_active(const CholWork &c);
+void launch_refine(const Stat &s, const CholWork &c, const double *p, const double *rad_all, hipStream_t q);
 
 }  // namespace bd

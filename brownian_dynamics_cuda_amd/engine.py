@@ -84,6 +84,7 @@ API = {
     "bd_set_chol_mode": ([C.c_void_p, C.c_int], C.c_int),
     "bd_set_chol_schedule": ([C.c_void_p, C.c_int], C.c_int),
     "bd_set_cell_list": ([C.c_void_p, C.c_int], C.c_int),
+    "bd_set_refine": ([C.c_void_p, C.c_int], C.c_int),
     "bd_cc_forward": ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
                       + [C.c_double] * 4 + [C.c_void_p] * 3, C.c_int),
     "bd_cc_backward": ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -354,6 +355,11 @@ class BDEngine:
         from 20000 tail and LH beads up) or bounding spheres of 16
         consecutive partners."""
         _check(self.lib.bd_set_cell_list(self.h, 1 if enable else 0), "bd_set_cell_list")
+
+    def set_refine(self, enable: bool):
+        """FP64 refinement of the noise displacement in the FP32 / BF16
+        factor modes (on by default; FP64 mode never refines)."""
+        _check(self.lib.bd_set_refine(self.h, 1 if enable else 0), "bd_set_refine")
 
     def chol_schedule(self) -> int:
         """Super-panel width in use (0: one block column per launch)."""

@@ -150,7 +150,9 @@ int bd_forces(bd_context *ctx, int terms, double *F, double *T, double *tF, doub
 int bd_factor(bd_context *ctx);
 int bd_get_factor(bd_context *ctx, double *L, int *neg_pivots);
 int bd_get_tensor(bd_context *ctx, double *D);
-/* rd = s2dt * L * p for the current factor (rd_cal, func_cuda.cu:634-647). */
+/* rd = s2dt * L * p for the current factor (rd_cal, func_cuda.cu:634-647);
+ * in the FP32 / BF16 modes with refinement on (bd_set_refine), the refined
+ * s2dt * L~ p' of the step. */
 int bd_noise_displacement(bd_context *ctx, const double *p, double *rd);
 int bd_set_chol_mode(bd_context *ctx, int chol_mode);
 
@@ -163,6 +165,18 @@ int bd_set_chol_mode(bd_context *ctx, int chol_mode);
  * bd_create chooses cells for fibers of 20000 tail and LH beads or more;
  * the environment variable BD_CELLS overrides that choice. */
 int bd_set_cell_list(bd_context *ctx, int enable);
+
+/* FP64 refinement of the noise displacement in the BD_CHOL_FP32 / BF16
+ * modes (the reference factors in FP64 throughout, func_cuda.cu:3513-3552,
+ * so this has no counterpart there).  With L~ the reduced-precision factor,
+ * rd = s2dt (L~ p + D u) / 2 with u = inv(L~)^T p: the first-order term of
+ * B p for B = L~ (inv(L~) D inv(L~)^T)^(1/2), a square root of D itself, so
+ * B B^T - D shrinks from O(E) to O(E^2) for a factor error E.  One blocked
+ * FP64 triangular solve with the factor and one matrix-free D u product
+ * from the positions D was assembled from.  enable = 1 (the default)
+ * refines in the reduced-precision modes; FP64 mode never refines.  The
+ * environment variable BD_REFINE sets the initial value. */
+int bd_set_refine(bd_context *ctx, int enable);
 
 /* Schedule of the blocked factor (no counterpart in the reference, which
  * calls cusolverDnDpotrf, func_cuda.cu:3526-3529).  panel_tiles = 0 (the
