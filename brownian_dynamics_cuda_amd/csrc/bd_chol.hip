@@ -986,6 +986,131 @@ __global__ void __launch_bounds__(256, 2) k_trail(double *A, int ld, int k0, int
     store_tile(A, ld, ib * TB, jb * TB, As, diag);
 }
 
+// Three-slice BF16 trailing update with the panel tiles split once.  The
+// on-the-fly form above (k_trail<2>) splits every operand fragment in every
+// workgroup, so each panel tile is split by all the workgroups of its tile
+// row and column and the update is bound by that VALU work.  Here k_split3
+// writes the three BF16 slices of every parked panel tile of the super-panel
+// once (slice s of the tile of row block i and panel column t at
+// sp[((i - j0) nb + t) 3 + s], 64 x 64 row-major), and k_trail_bf streams
+// them through LDS (rows padded to kSplitPitch, so the 16-byte fragment reads
+// of a 16-lane row group fall in distinct banks) into the same six
+// v_mfma_f32_16x16x32_bf16 products in the same order, lane mapping and
+// split: the result is the on-the-fly update bit for bit.
+constexpr int kSplitPitch = 72;
+constexpr size_t kSplitTile = (size_t)TB * TB;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) k_split3(const double *A, int ld, int k0, int nb, int j0, __bf16 *sp) {
+    const int i = j0 + blockIdx.x / nb, t = blockIdx.x % nb;
+    const double *src = A + (size_t)(k0 + t) * TB * ld + (size_t)i * TB;
+    __bf16 *dst = sp + (size_t)blockIdx.x * 3 * kSplitTile;
+#pragma unroll
+    for (int e = threadIdx.x * 8; e < TB * TB; e += 256 * 8) {
+        const int r = e / TB, c = e % TB;
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q += 2) {
+            const dbl2 x = *reinterpret_cast<const dbl2 *>(src + (size_t)r * ld + c + q);
+            v[q] = x.x;
+            v[q + 1] = x.y;
+        }
+        bf8 h, m, l;
+        split3(v, h, m, l);
+        *reinterpret_cast<bf8 *>(dst + e) = h;
+        *reinterpret_cast<bf8 *>(dst + kSplitTile + e) = m;
+        *reinterpret_cast<bf8 *>(dst + 2 * kSplitTile + e) = l;
+    }
+}
+
+// The three slices of one split tile (global) into registers: thread t
+// takes the 16-byte pieces t and t + 256 of each slice.
+__device__ __forceinline__ void split_to_regs(const __bf16 *src, u32x4 (&v)[6]) {
+#pragma unroll
+    for (int s = 0; s < 3; s++)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            v[2 * s + h] = *reinterpret_cast<const u32x4 *>(src + s * kSplitTile + 8 * (threadIdx.x + 256 * h));
+}
+__device__ __forceinline__ void split_regs_to_lds(const u32x4 (&v)[6], __bf16 *S) {
+#pragma unroll
+    for (int s = 0; s < 3; s++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int e = 8 * (threadIdx.x + 256 * h), r = e / TB, c = e % TB;
+            *reinterpret_cast<u32x4 *>(S + (s * TB + r) * kSplitPitch + c) = v[2 * s + h];
+        }
+}
+
+__global__ void __launch_bounds__(256, 2) k_trail_bf(double *A, int ld, const __bf16 *sp, int nb, int j0, int nt) {
+    // A's and B's slices; the target tile is staged over both at the end
+    __shared__ __attribute__((aligned(16))) __bf16 Sab[2 * 3 * TB * kSplitPitch];
+    static_assert(sizeof(Sab) >= sizeof(double) * TB * LDP, "C staging fits the slice buffers");
+    __bf16 *const Sa = Sab, *const Sb = Sab + 3 * TB * kSplitPitch;
+    int SI, SJ;
+    tri_index(blockIdx.x >> 6, SI, SJ);
+    const int ib = j0 + 8 * SI + ((blockIdx.x >> 3) & 7), jb = j0 + 8 * SJ + (blockIdx.x & 7);
+    if (ib >= nt || jb > ib) return;
+    const bool diag = ib == jb;
+    const __bf16 *pa = sp + (size_t)(ib - j0) * nb * 3 * kSplitTile;
+    const __bf16 *pb = sp + (size_t)(jb - j0) * nb * 3 * kSplitTile;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32, i16 = lane & 15, k8 = lane >> 4;
+    flt4 acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
+    u32x4 va[6], vb[6];
+    constexpr int PER = TB * TB / 2 / NTHR_T;
+    dbl2 vc[PER];
+    split_to_regs(pa, va);
+    if (!diag) split_to_regs(pb, vb);
+    const __bf16 *Bsrc = diag ? Sa : Sb;
+    for (int t = 0; t < nb; t++) {
+        split_regs_to_lds(va, Sa);
+        if (!diag) split_regs_to_lds(vb, Sb);
+        __syncthreads();
+        if (t + 1 < nb) {
+            split_to_regs(pa + (size_t)(t + 1) * 3 * kSplitTile, va);
+            if (!diag) split_to_regs(pb + (size_t)(t + 1) * 3 * kSplitTile, vb);
+        } else {
+            tile_to_regs(A + (size_t)ib * TB * ld + jb * TB, ld, vc);
+        }
+#pragma unroll
+        for (int kk = 0; kk < TB; kk += 32) {
+            bf8 ah[2], am[2], al[2], bh[2], bm[2], bl[2];
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int ra = wr + 16 * q + i16, rb = wc + 16 * q + i16, c = kk + 8 * k8;
+                ah[q] = *reinterpret_cast<const bf8 *>(Sa + (0 * TB + ra) * kSplitPitch + c);
+                am[q] = *reinterpret_cast<const bf8 *>(Sa + (1 * TB + ra) * kSplitPitch + c);
+                al[q] = *reinterpret_cast<const bf8 *>(Sa + (2 * TB + ra) * kSplitPitch + c);
+                bh[q] = *reinterpret_cast<const bf8 *>(Bsrc + (0 * TB + rb) * kSplitPitch + c);
+                bm[q] = *reinterpret_cast<const bf8 *>(Bsrc + (1 * TB + rb) * kSplitPitch + c);
+                bl[q] = *reinterpret_cast<const bf8 *>(Bsrc + (2 * TB + rb) * kSplitPitch + c);
+            }
+#pragma unroll
+            for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+                for (int ni = 0; ni < 2; ni++) {
+                    flt4 c = acc[mi][ni];
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], bh[ni], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[mi], bm[ni], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bl[ni], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[mi], bh[ni], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bm[ni], c, 0, 0, 0);
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bh[ni], c, 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    double *Cs = reinterpret_cast<double *>(Sab);
+    regs_to_lds(vc, Cs);
+    __syncthreads();
+    acc_to_lds(Cs, acc, -1.0, false);
+    __syncthreads();
+    store_tile(A, ld, ib * TB, jb * TB, Cs, diag);
+}
+
 // Factor and inverse of the diagonal tile (d, d) by one workgroup (the
 // tile after a super-panel's trailing update, or the first tile of a
 // caller's matrix); clear != 0 also restarts the negative-pivot count.
@@ -1191,9 +1316,17 @@ void chol_column(const CholWork &c, int kb, int jcols, hipStream_t q) {
 template <int MODE>
 void chol_trail(const CholWork &c, int k0, int nb, hipStream_t q) {
     const int nt = c.mpad / TB, j0 = k0 + nb, SM = (nt - j0 + 7) / 8;
+    if (MODE == 2 && c.sp && nb <= c.sp_nb) {
+        __bf16 *sp = static_cast<__bf16 *>(c.sp);
+        k_split3<<<(nt - j0) * nb, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, sp);
+        k_trail_bf<<<SM * (SM + 1) / 2 * 64, 256, 0, q>>>(c.A, c.ld, sp, nb, j0, nt);
+        return;
+    }
     k_trail<MODE><<<SM * (SM + 1) / 2 * 64, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, nt);
 }
 }  // namespace
+
+size_t split_bytes(int ld, int nb) { return (size_t)(ld / TB) * nb * 3 * kSplitTile * sizeof(__bf16); }
 
 int chol_panel_tiles(const CholWork &c) {
     const int nt = c.mpad / TB;

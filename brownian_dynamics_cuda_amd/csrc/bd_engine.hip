@@ -552,6 +552,12 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     r |= c->alloc(&c->chol.xs, (size_t)c->chol.ld);
     r |= c->alloc(&c->chol.rv, 2 * (size_t)c->chol.ld);
     r |= c->alloc(&c->chol.pr, (size_t)c->chol.ld);
+    if (!getenv("BD_BF16_SPLIT") || atoi(getenv("BD_BF16_SPLIT")) != 0) {
+        double *spd = nullptr;
+        c->chol.sp_nb = 8;
+        r |= c->alloc(&spd, (split_bytes(c->chol.ld, c->chol.sp_nb) + 7) / 8);
+        c->chol.sp = spd;
+    }
     r |= c->alloc(&c->rd, 3 * nD_cap);
     r |= c->alloc(&c->p, 3 * nD_cap);
     r |= c->alloc(&c->rr, 3 * n);

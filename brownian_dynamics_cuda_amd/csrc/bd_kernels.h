@@ -84,7 +84,13 @@ struct CholWork {
     // rv, two work vectors of ld doubles; pr, D inv(L~)^T p (ld)
     int refine;
     double *xs, *rv, *pr;
+    // BF16 mode: the super-panels' panel tiles split once into three BF16
+    // slices (split_bytes(ld, sp_nb) bytes; null: split on the fly)
+    void *sp;
+    int sp_nb;
 };
+// bytes of the BF16 split buffer for leading dimension ld and panels of nb
+size_t split_bytes(int ld, int nb);
 // super-panel width the factor of c uses (0: one block column at a time)
 int chol_panel_tiles(const CholWork &c);
 // D into c.A; also clears the negative-pivot count of the factor that

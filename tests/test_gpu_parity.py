@@ -363,6 +363,26 @@ def test_super_panel_schedule_factor(panel):
         assert e <= 1e-6 and rel_err(Ls[mode], L0) <= 1e-5
 
 
+@pytest.mark.parametrize("panel", [2, 3, 8])
+def test_bf16_presplit_equals_on_the_fly(panel, monkeypatch):
+    """The BF16 trailing update from panel tiles split once per super-panel
+    (k_split3 + k_trail_bf, the default) against the form that splits every
+    operand fragment in every workgroup (k_trail<2>, BD_BF16_SPLIT=0): the
+    same products in the same order, so the same factor bit for bit (6-core
+    deck, full hydrodynamic set, D 1404 x 1404, 22 block columns)."""
+    fib, st = deck("small", hi_all=1)
+    Ls = {}
+    for split in ("1", "0"):
+        monkeypatch.setenv("BD_BF16_SPLIT", split)
+        with E.BDEngine(fib, chol_mode=E.CHOL_BF16, use_graph=False) as eng:
+            eng.set_chol_schedule(panel)
+            eng.set_state(st, rebuild=True)
+            eng.factor()
+            Ls[split], neg = eng.get_factor()
+            assert neg == 0
+    assert np.array_equal(Ls["1"], Ls["0"])
+
+
 def _mixed_step_errors(fib, st, rr, p, modes):
     """Displacement of one step and ||L L^T - D|| / ||D|| for each factor
     mode, from the same state and noise."""
