@@ -445,8 +445,9 @@ def test_frequency_rp_against_oracle_reuse(hi):
 
 @pytest.mark.parametrize("kind", ["small", "cfg2"])
 def test_cell_list_matches_chunk_search(kind):
-    """The 7 nm cell grids (bd_set_cell_list(1), the default) and the chunk
-    spheres (0) evaluate the same pair set: forces within summation-order
+    """The 7 nm cell grids (bd_set_cell_list(1), forced on here; the default
+    from 20000 tail and LH beads up) and the chunk spheres (0) evaluate the
+    same pair set: forces within summation-order
     rounding of each other and of the oracle; one graph step each."""
     fib, st = deck(kind) if kind != "cfg2" else Fb.make_config(2)
     orc = O.Oracle(fib)
