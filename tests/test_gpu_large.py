@@ -13,7 +13,9 @@ beads) and the 500-nucleosome fiber (D 10482 x 10482), through the C-ABI.
   * graph replay equal to eager launches bit for bit;
   * the precision forms (FP32 and three-slice BF16 trailing updates): the
     displacement of one step within 1e-6 of the FP64 step from the same
-    state and noise, and ||L L^T - D|| / ||D|| (probed) within 1e-6.
+    state and noise, and ||L L^T - D|| / ||D|| (probed) within 1e-6;
+  * the FP64 refinement of B xi at 500 nucleosomes against its formula and
+    its covariance bound.
 Reference: main_cuda (func_cuda.cu:3434-3680), func.cpp force terms."""
 import numpy as np
 import pytest
@@ -152,3 +154,39 @@ def test_precision_forms(cfg):
         rel = np.linalg.norm(disp[mode] - disp[E.CHOL_FP64]) / np.linalg.norm(disp[E.CHOL_FP64])
         print(f"cfg{cfg} mode {mode}: displacement {rel:.2e}, factor {ferr[mode]:.2e}", flush=True)
         assert rel <= 1e-6 and ferr[mode] <= 1e-6
+
+
+@pytest.mark.parametrize("mode", [E.CHOL_FP32, E.CHOL_BF16])
+def test_refined_noise_cfg3(mode):
+    """The FP64 refinement of B xi at 500 nucleosomes (164 block columns:
+    super-panels, the pre-split BF16 update, 82 two-block solve launches):
+    the device noise against rd = s2dt (L~ p + D u) / 2, u = inv(L~)^T p, from
+    the device factor and tensor, and the covariance of the refined map,
+    ||B B^T v - D v|| / ||D v|| on random probes (B = L~ (I + X / 2),
+    X = inv(L~) D inv(L~)^T - I, applied through triangular solves)."""
+    from scipy.linalg import solve_triangular
+    print(f"case: refined noise at cfg3, mode {mode}", flush=True)
+    fib, st = Fb.make_config(3)
+    with E.BDEngine(fib, chol_mode=mode, use_graph=False) as eng:
+        eng.set_state(st, rebuild=True)
+        D = eng.get_tensor()
+        eng.factor()
+        L, neg = eng.get_factor()
+        p = np.random.default_rng(17).standard_normal(3 * (fib.n + fib.n_tail + fib.n_LH))
+        rd = eng.noise_displacement(p)
+    assert neg == 0
+    m = D.shape[0]
+    u = solve_triangular(L.T, p[:m], lower=False)
+    exp = fib.phys.s2dt * 0.5 * (L @ p[:m] + D @ u)
+    e = rel_err(rd[:m], exp)
+    g = np.random.default_rng(5)
+    cov = 0.0
+    for _ in range(2):
+        v = g.standard_normal(m)
+        y = L.T @ v
+        bt = 0.5 * (y + solve_triangular(L, D @ solve_triangular(L.T, y, lower=False), lower=True))
+        bbt = 0.5 * (L @ bt + D @ solve_triangular(L.T, bt, lower=False))
+        Dv = D @ v
+        cov = max(cov, float(np.linalg.norm(bbt - Dv) / np.linalg.norm(Dv)))
+    print(f"  rd vs formula {e:.2e}, refined covariance {cov:.2e}, factor {probe_err(L, D):.2e}", flush=True)
+    assert e <= 1e-11 and cov <= 1e-12
