@@ -991,23 +991,31 @@ __global__ void __launch_bounds__(256, 2) k_trail(double *A, int ld, int k0, int
 // workgroup, so each panel tile is split by all the workgroups of its tile
 // row and column and the update is bound by that VALU work.  Here k_split3
 // writes the three BF16 slices of every parked panel tile of the super-panel
-// once (slice s of the tile of row block i and panel column t at
-// sp[((i - j0) nb + t) 3 + s], 64 x 64 row-major), and k_trail_bf streams
-// them through LDS (rows padded to kSplitPitch, so the 16-byte fragment reads
-// of a 16-lane row group fall in distinct banks) into the same six
-// v_mfma_f32_16x16x32_bf16 products in the same order, lane mapping and
-// split: the result is the on-the-fly update bit for bit.
-constexpr int kSplitPitch = 72;
+// once, in the operand order of v_mfma_f32_16x16x32_bf16: fragment (rb, ks)
+// of a slice (16 rows from 16 rb, 32 K from 32 ks) is 64 lanes x 16 bytes,
+// lane l holding row 16 rb + l % 16 and K entries 32 ks + 8 (l / 16) .. +7.
+// Slice s of the tile of row block i and panel column t sits at
+// sp[((i - j0) nb + t) 3 + s].  k_trail_bf copies a panel column's 48
+// fragments (A and B operands, three slices) straight from global memory
+// into LDS (global_load_lds_dwordx4: lane l's 16 bytes land at the fragment's
+// slot + 16 l, so a fragment is one contiguous kilobyte in LDS as in global
+// memory and every wavefront read of it is conflict-free), each wavefront
+// reads its fragments into registers, and the next column's copy runs while
+// they are multiplied.  The six products, their order, the lane mapping and
+// the split are those of k_trail<2>: the same factor bit for bit.
 constexpr size_t kSplitTile = (size_t)TB * TB;
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ size_t frag_off(int rb, int ks, int lane) { return (size_t)((rb * 2 + ks) * 64 + lane) * 8; }
 
+// one workgroup per (row block, panel column) tile; wavefront w writes the
+// fragments of row block w
 __global__ void __launch_bounds__(256) k_split3(const double *A, int ld, int k0, int nb, int j0, __bf16 *sp) {
     const int i = j0 + blockIdx.x / nb, t = blockIdx.x % nb;
     const double *src = A + (size_t)(k0 + t) * TB * ld + (size_t)i * TB;
     __bf16 *dst = sp + (size_t)blockIdx.x * 3 * kSplitTile;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-    for (int e = threadIdx.x * 8; e < TB * TB; e += 256 * 8) {
-        const int r = e / TB, c = e % TB;
+    for (int ks = 0; ks < 2; ks++) {
+        const int r = 16 * w + (lane & 15), c = 32 * ks + 8 * (lane >> 4);
         double v[8];
 #pragma unroll
         for (int q = 0; q < 8; q += 2) {
@@ -1017,98 +1025,115 @@ __global__ void __launch_bounds__(256) k_split3(const double *A, int ld, int k0,
         }
         bf8 h, m, l;
         split3(v, h, m, l);
-        *reinterpret_cast<bf8 *>(dst + e) = h;
-        *reinterpret_cast<bf8 *>(dst + kSplitTile + e) = m;
-        *reinterpret_cast<bf8 *>(dst + 2 * kSplitTile + e) = l;
+        const size_t o = frag_off(w, ks, lane);
+        *reinterpret_cast<bf8 *>(dst + o) = h;
+        *reinterpret_cast<bf8 *>(dst + kSplitTile + o) = m;
+        *reinterpret_cast<bf8 *>(dst + 2 * kSplitTile + o) = l;
     }
 }
 
-// The three slices of one split tile (global) into registers: thread t
-// takes the 16-byte pieces t and t + 256 of each slice.
-__device__ __forceinline__ void split_to_regs(const __bf16 *src, u32x4 (&v)[6]) {
+// The LDS stage: 48 fragment slots of a kilobyte, slot (op, s, rb, ks) for
+// operand op (0: the row block's tiles, 1: the column block's; a diagonal
+// tile loads only op 0), slice s, row block rb, K step ks.  Wavefront w
+// copies slots w NLD .. w NLD + NLD - 1.
+typedef __attribute__((address_space(3))) void lds_void;
+__device__ __forceinline__ int stage_frag(int op, int s, int rb, int ks) { return (op * 3 + s) * 8 + rb * 2 + ks; }
+template <int NLD>
+__device__ __forceinline__ void issue_stage(const __bf16 *pa, const __bf16 *pb, int t, __bf16 *S, int w, int lane) {
 #pragma unroll
-    for (int s = 0; s < 3; s++)
-#pragma unroll
-        for (int h = 0; h < 2; h++)
-            v[2 * s + h] = *reinterpret_cast<const u32x4 *>(src + s * kSplitTile + 8 * (threadIdx.x + 256 * h));
+    for (int j = 0; j < NLD; j++) {
+        const int f = w * NLD + j, op = f / 24, s = (f / 8) % 3, fr = f % 8;
+        const __bf16 *src = (op ? pb : pa) + (size_t)t * 3 * kSplitTile + s * kSplitTile + (size_t)fr * 512;
+        __builtin_amdgcn_global_load_lds(src + lane * 8, (lds_void *)(S + (size_t)f * 512), 16, 0, 0);
+    }
 }
-__device__ __forceinline__ void split_regs_to_lds(const u32x4 (&v)[6], __bf16 *S) {
+template <int BOP>
+__device__ __forceinline__ void stage_read(const __bf16 *S, int wr, int wc, int lane, bf8 (&fa)[2][3][2], bf8 (&fb)[2][3][2]) {
 #pragma unroll
-    for (int s = 0; s < 3; s++)
+    for (int ks = 0; ks < 2; ks++)
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int e = 8 * (threadIdx.x + 256 * h), r = e / TB, c = e % TB;
-            *reinterpret_cast<u32x4 *>(S + (s * TB + r) * kSplitPitch + c) = v[2 * s + h];
-        }
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int s = 0; s < 3; s++) {
+                fa[ks][s][q] = *reinterpret_cast<const bf8 *>(S + (size_t)stage_frag(0, s, wr / 16 + q, ks) * 512 + lane * 8);
+                fb[ks][s][q] = *reinterpret_cast<const bf8 *>(S + (size_t)stage_frag(BOP, s, wc / 16 + q, ks) * 512 + lane * 8);
+            }
 }
-
+__device__ __forceinline__ void frags_mma(const bf8 (&fa)[2][3][2], const bf8 (&fb)[2][3][2], flt4 (&acc)[2][2]) {
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+        for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+            for (int ni = 0; ni < 2; ni++) {
+                flt4 c = acc[mi][ni];
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ks][2][mi], fb[ks][0][ni], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ks][1][mi], fb[ks][1][ni], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ks][0][mi], fb[ks][2][ni], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ks][1][mi], fb[ks][0][ni], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ks][0][mi], fb[ks][1][ni], c, 0, 0, 0);
+                acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ks][0][mi], fb[ks][0][ni], c, 0, 0, 0);
+            }
+}
+// Per panel column t: every wavefront waits for its own copies (vmcnt(0);
+// the LDS writes of a copy are done when it retires), a barrier makes the
+// stage whole, each wavefront reads its 24 fragments into registers
+// (lgkmcnt(0)), a second barrier frees the stage, and the copy of column
+// t + 1 then runs beside the 48 products of column t.  The compiler does not
+// order the copies against these LDS reads, hence the explicit waits.
+template <bool DIAG>
+__device__ __forceinline__ void trail_bf_body(const __bf16 *pa, const __bf16 *pb, int nb, __bf16 *St, int w, int lane,
+                                              int wr, int wc, flt4 (&acc)[2][2]) {
+    constexpr int NLD = DIAG ? 6 : 12, BOP = DIAG ? 0 : 1;
+    bf8 fa[2][3][2], fb[2][3][2];
+    issue_stage<NLD>(pa, pb, 0, St, w, lane);
+    for (int t = 0; t < nb; t++) {
+        __builtin_amdgcn_s_waitcnt(0xF70);   // vmcnt(0)
+        __builtin_amdgcn_s_barrier();
+        stage_read<BOP>(St, wr, wc, lane, fa, fb);
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+        __builtin_amdgcn_s_barrier();
+        if (t + 1 < nb) issue_stage<NLD>(pa, pb, t + 1, St, w, lane);
+        frags_mma(fa, fb, acc);
+    }
+}
 __global__ void __launch_bounds__(256, 2) k_trail_bf(double *A, int ld, const __bf16 *sp, int nb, int j0, int nt) {
-    // A's and B's slices; the target tile is staged over both at the end
-    __shared__ __attribute__((aligned(16))) __bf16 Sab[2 * 3 * TB * kSplitPitch];
-    static_assert(sizeof(Sab) >= sizeof(double) * TB * LDP, "C staging fits the slice buffers");
-    __bf16 *const Sa = Sab, *const Sb = Sab + 3 * TB * kSplitPitch;
+    __shared__ __attribute__((aligned(16))) __bf16 St[48 * 512];
     int SI, SJ;
     tri_index(blockIdx.x >> 6, SI, SJ);
     const int ib = j0 + 8 * SI + ((blockIdx.x >> 3) & 7), jb = j0 + 8 * SJ + (blockIdx.x & 7);
     if (ib >= nt || jb > ib) return;
     const bool diag = ib == jb;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32, i16 = lane & 15, k4 = lane >> 4;
     const __bf16 *pa = sp + (size_t)(ib - j0) * nb * 3 * kSplitTile;
     const __bf16 *pb = sp + (size_t)(jb - j0) * nb * 3 * kSplitTile;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int wr = (w >> 1) * 32, wc = (w & 1) * 32, i16 = lane & 15, k8 = lane >> 4;
     flt4 acc[2][2];
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++) acc[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
-    u32x4 va[6], vb[6];
-    constexpr int PER = TB * TB / 2 / NTHR_T;
-    dbl2 vc[PER];
-    split_to_regs(pa, va);
-    if (!diag) split_to_regs(pb, vb);
-    const __bf16 *Bsrc = diag ? Sa : Sb;
-    for (int t = 0; t < nb; t++) {
-        split_regs_to_lds(va, Sa);
-        if (!diag) split_regs_to_lds(vb, Sb);
-        __syncthreads();
-        if (t + 1 < nb) {
-            split_to_regs(pa + (size_t)(t + 1) * 3 * kSplitTile, va);
-            if (!diag) split_to_regs(pb + (size_t)(t + 1) * 3 * kSplitTile, vb);
-        } else {
-            tile_to_regs(A + (size_t)ib * TB * ld + jb * TB, ld, vc);
-        }
+    if (diag) trail_bf_body<true>(pa, pb, nb, St, w, lane, wr, wc, acc);
+    else trail_bf_body<false>(pa, pb, nb, St, w, lane, wr, wc, acc);
+    // C -= acc in place: all sixteen loads first, then the stores
+    double cv[2][2][4];
 #pragma unroll
-        for (int kk = 0; kk < TB; kk += 32) {
-            bf8 ah[2], am[2], al[2], bh[2], bm[2], bl[2];
+    for (int mi = 0; mi < 2; mi++)
 #pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const int ra = wr + 16 * q + i16, rb = wc + 16 * q + i16, c = kk + 8 * k8;
-                ah[q] = *reinterpret_cast<const bf8 *>(Sa + (0 * TB + ra) * kSplitPitch + c);
-                am[q] = *reinterpret_cast<const bf8 *>(Sa + (1 * TB + ra) * kSplitPitch + c);
-                al[q] = *reinterpret_cast<const bf8 *>(Sa + (2 * TB + ra) * kSplitPitch + c);
-                bh[q] = *reinterpret_cast<const bf8 *>(Bsrc + (0 * TB + rb) * kSplitPitch + c);
-                bm[q] = *reinterpret_cast<const bf8 *>(Bsrc + (1 * TB + rb) * kSplitPitch + c);
-                bl[q] = *reinterpret_cast<const bf8 *>(Bsrc + (2 * TB + rb) * kSplitPitch + c);
+        for (int ni = 0; ni < 2; ni++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = wr + 16 * mi + 4 * k4 + v, c = wc + 16 * ni + i16;
+                cv[mi][ni][v] = A[(size_t)(ib * TB + r) * ld + jb * TB + c];
             }
 #pragma unroll
-            for (int mi = 0; mi < 2; mi++)
+    for (int mi = 0; mi < 2; mi++)
 #pragma unroll
-                for (int ni = 0; ni < 2; ni++) {
-                    flt4 c = acc[mi][ni];
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], bh[ni], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[mi], bm[ni], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bl[ni], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[mi], bh[ni], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bm[ni], c, 0, 0, 0);
-                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bh[ni], c, 0, 0, 0);
-                }
-        }
-        __syncthreads();
-    }
-    double *Cs = reinterpret_cast<double *>(Sab);
-    regs_to_lds(vc, Cs);
-    __syncthreads();
-    acc_to_lds(Cs, acc, -1.0, false);
-    __syncthreads();
-    store_tile(A, ld, ib * TB, jb * TB, Cs, diag);
+        for (int ni = 0; ni < 2; ni++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = wr + 16 * mi + 4 * k4 + v, c = wc + 16 * ni + i16;
+                if (diag && c > r) continue;
+                A[(size_t)(ib * TB + r) * ld + jb * TB + c] = cv[mi][ni][v] + -1.0 * (double)acc[mi][ni][v];
+            }
 }
 
 // Factor and inverse of the diagonal tile (d, d) by one workgroup (the
