@@ -21,7 +21,9 @@
 // and the trailing matrix then receives the whole panel in one launch
 // (k_trail: each tile accumulates K = 64 NB in registers before its one
 // read-modify-write), so the trailing matrix streams through HBM once per
-// NB block columns rather than once per block column.
+// NB block columns rather than once per block column.  In the FP32 and BF16
+// modes only that trailing update takes the reduced precision; the panel's
+// own columns stay FP64.
 #include <algorithm>
 
 #include "bd_dev.h"
@@ -1363,10 +1365,10 @@ int chol_panel_tiles(const CholWork &c) {
 void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_arg, hipEvent_t gate, int gate_kb) {
     const int nt = c.mpad / TB;
     auto M_ = [&](int id, int ph) { if (mark) mark(mark_arg, id, ph); };
-    auto column = [&](int kb, int jcols) {
+    auto column = [&](int kb, int jcols, int mode) {
         M_(K_CHOL_STEP, 0);
-        if (c.mode == 0) chol_column<0>(c, kb, jcols, q);
-        else if (c.mode == 1) chol_column<1>(c, kb, jcols, q);
+        if (mode == 0) chol_column<0>(c, kb, jcols, q);
+        else if (mode == 1) chol_column<1>(c, kb, jcols, q);
         else chol_column<2>(c, kb, jcols, q);
         M_(K_CHOL_STEP, 1);
         if (gate && kb == gate_kb) (void)hipEventRecord(gate, q);
@@ -1374,10 +1376,12 @@ void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_a
     const int NB = chol_panel_tiles(c);
     int k0 = 0;
     if (NB > 0) {
-        // super-panels while a whole one and a trailing matrix remain
+        // super-panels while a whole one and a trailing matrix remain; the
+        // panel's own columns are factored in FP64 in every mode (the
+        // reduced-precision modes apply to the trailing update k_trail)
         for (; k0 + NB < nt; k0 += NB) {
             const int kend = k0 + NB;
-            for (int kb = k0; kb < kend; kb++) column(kb, kend - 1 - kb);
+            for (int kb = k0; kb < kend; kb++) column(kb, kend - 1 - kb, 0);
             M_(K_CHOL_TRAIL, 0);
             if (c.mode == 0) chol_trail<0>(c, k0, NB, q);
             else if (c.mode == 1) chol_trail<1>(c, k0, NB, q);
@@ -1389,7 +1393,7 @@ void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_a
         }
     }
     // one launch (or two) per block column, each updating the whole trailing matrix
-    for (int kb = k0; kb + 1 < nt; kb++) column(kb, nt - kb - 1);
+    for (int kb = k0; kb + 1 < nt; kb++) column(kb, nt - kb - 1, c.mode);
     if (gate && gate_kb >= nt - 1) (void)hipEventRecord(gate, q);
 }
 
