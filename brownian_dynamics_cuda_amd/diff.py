@@ -792,4 +792,4 @@ class DiffBD:
         L = self.end_to_end2(xT) if loss == "end_to_end" else self.energy(xT, prm)
         keys = ("q_core", "t_q", "h", "g", "s")
         grads = torch.autograd.grad(L, [prm[k] for k in keys])
-        return float(L), {k: gr.detach().cpu().numpy() for k, gr in zip(keys, grads)}
+        return float(L.detach()), {k: gr.detach().cpu().numpy() for k, gr in zip(keys, grads)}
