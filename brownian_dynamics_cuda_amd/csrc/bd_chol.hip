@@ -891,13 +891,13 @@ __global__ void __launch_bounds__(256) k_panel(double *A, int ld, int kb, const 
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(256) k_update(double *A, int ld, int kb, double *Linv, int *neg) {
+__global__ void __launch_bounds__(256) k_update(double *A, int ld, int kb, int jcols, double *Linv, int *neg) {
     __shared__ double As[TB * LDP], Bs[TB * LDP], Cs[TB * LDP];
     int lb = blockIdx.x;  // XCD-contiguous runs of tiles, as in k_step
     const int G8 = gridDim.x / 8;
     if (lb < 8 * G8) lb = (lb % 8) * G8 + lb / 8;
     int ii, jj;
-    tri_index(lb, ii, jj);
+    trap_index(lb, jcols, ii, jj);
     const int ib = kb + 1 + ii, jb = kb + 1 + jj;
     const bool diag = ib == jb;
     {
@@ -1330,13 +1330,13 @@ namespace {
 template <int MODE>
 void chol_column(const CholWork &c, int kb, int jcols, hipStream_t q) {
     const int nt = c.mpad / TB, M = nt - kb - 1;
+    const int grid = jcols * (jcols + 1) / 2 + (M - jcols) * jcols;
     if (jcols == 0) {
         k_panel<<<M, 256, 0, q>>>(c.A, c.ld, kb, c.Linv);
-    } else if (jcols == M && M * (M + 1) / 2 >= c.split_tiles) {
+    } else if (grid >= c.split_tiles) {
         k_panel<<<M, 256, 0, q>>>(c.A, c.ld, kb, c.Linv);
-        k_update<MODE><<<M * (M + 1) / 2, 256, 0, q>>>(c.A, c.ld, kb, c.Linv, c.neg);
+        k_update<MODE><<<grid, 256, 0, q>>>(c.A, c.ld, kb, jcols, c.Linv, c.neg);
     } else {
-        const int grid = jcols * (jcols + 1) / 2 + (M - jcols) * jcols;
         k_step<MODE><<<grid, 256, 0, q>>>(c.A, c.ld, kb, jcols, c.Linv, c.neg);
     }
 }
