@@ -96,7 +96,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs one fiber on one GPU; --gpus {args.gpus} is not supported "
-              "(sharding is described in DESIGN.md)", file=sys.stderr)
+              "(several GPUs would take one fiber each, or split a fiber's force pass by "
+              "receiver ranges)", file=sys.stderr)
         sys.exit(2)
     cfg = args.config
     fib, st = Fb.make_config(cfg)
