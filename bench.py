@@ -28,6 +28,7 @@ from brownian_dynamics_cuda_amd import fiber as Fb  # noqa: E402
 
 PEAK_FP64 = 78.6e12  # MI355X FP64 (matrix and vector), FLOP/s
 PEAK_FP32_MFMA = 157.3e12
+PEAK_BF16_MFMA = 2.5e15  # dense
 PEAK_HBM = 8.0e12    # B/s
 
 
@@ -40,7 +41,10 @@ def parse():
     ap.add_argument("--timing-steps", type=int, default=5)
     ap.add_argument("--host-io-steps", type=int, default=50,
                     help="steps timed through bd_step with host noise buffers (PCIe-inclusive rate)")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="reference CPU steps to time (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="reference CPU steps to time (0: skip; default 2 for the 6- to 100-nucleosome "
+                         "fibers, 0 for the 500- and 1000-nucleosome ones, where one CPU step "
+                         "takes many minutes)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--chol", choices=("fp64", "fp32", "bf16"), default="fp64",
                     help="factor arithmetic of the timed steps (the headline is fp64; the "
@@ -162,28 +166,36 @@ def main():
     # roofline of the dominant kernel
     tot_ms, launches = kt[dominant]
     avg_s = tot_ms / launches / 1e3
+    # super-panel trailing updates: the lower triangle of each trailing
+    # matrix (m_s rows) takes a rank-K update, K = 64 NB: K m_s (m_s + 1)
+    # FLOPs per launch; the block-column launches carry the rest of n^3/3
+    nb = eng.chol_schedule()
+    nt = (n_hi3 + 63) // 64
+    trail_fl = []
+    k0 = 0
+    while nb and k0 + nb < nt:
+        ms = n_hi3 - 64 * (k0 + nb)
+        trail_fl.append(64.0 * nb * ms * (ms + 1))
+        k0 += nb
     if dominant in ("chol_step", "potrf"):
-        # the factorisation's algorithmic FLOPs, n^3/3, spread over the
-        # block-column launches (one per 64 columns after the first)
-        per_launch = chol_flops(n_hi3) / (launches / args.timing_steps)
+        # the factorisation's algorithmic FLOPs less the trailing updates',
+        # spread over the block-column launches (one per 64 columns after
+        # the first); with super-panels these are the FP64 panel columns
+        per_launch = (chol_flops(n_hi3) - sum(trail_fl)) / (launches / args.timing_steps)
+        pipe = "FP64 MFMA" if args.chol == "fp64" or trail_fl else f"FP64 + {args.chol.upper()} MFMA"
         roof = dict(bound="mfma", achieved=per_launch / avg_s / 1e12, peak=PEAK_FP64 / 1e12,
-                    unit="TFLOP/s", kernel=dominant, pipe="FP64 MFMA",
+                    unit="TFLOP/s", kernel=dominant, pipe=pipe,
                     flop_per_launch=per_launch, avg_launch_us=avg_s * 1e6)
     elif dominant == "chol_trail":
-        # super-panel trailing updates: the lower triangle of each trailing
-        # matrix (m_s rows) takes a rank-K update, K = 64 NB: K m_s (m_s + 1)
-        # FLOPs; averaged over the launches of one factorisation
-        nb = eng.chol_schedule()
-        nt = (n_hi3 + 63) // 64
-        fl = []
-        k0 = 0
-        while nb and k0 + nb < nt:
-            ms = n_hi3 - 64 * (k0 + nb)
-            fl.append(64.0 * nb * ms * (ms + 1))
-            k0 += nb
-        per_launch = sum(fl) / len(fl)
-        roof = dict(bound="mfma", achieved=per_launch / avg_s / 1e12, peak=PEAK_FP64 / 1e12,
-                    unit="TFLOP/s", kernel=dominant, pipe="FP64 MFMA",
+        # averaged over the launches of one factorisation, priced against the
+        # matrix peak of the mode's arithmetic; the three-slice BF16 form
+        # makes six BF16 products per algorithmic FLOP
+        per_launch = sum(trail_fl) / len(trail_fl)
+        peak, pipe, mult = {"fp64": (PEAK_FP64, "FP64 MFMA", 1.0),
+                            "fp32": (PEAK_FP32_MFMA, "FP32 MFMA", 1.0),
+                            "bf16": (PEAK_BF16_MFMA, "BF16 MFMA, six products per FLOP", 6.0)}[args.chol]
+        roof = dict(bound="mfma", achieved=mult * per_launch / avg_s / 1e12, peak=peak / 1e12,
+                    unit="TFLOP/s", kernel=dominant, pipe=pipe,
                     flop_per_launch=per_launch, avg_launch_us=avg_s * 1e6, panel_tiles=nb)
     else:
         # Debye-Hueckel charge sums: 38 FLOPs per charge pair, FP64 on the
@@ -194,8 +206,11 @@ def main():
                     unit="TFLOP/s", kernel=dominant, pipe="FP64 VALU",
                     flop_per_launch=per_launch, avg_launch_us=avg_s * 1e6)
     roof["frac"] = roof["achieved"] / roof["peak"]
-    tfile = args.traffic_file or os.path.join(ROOT, "profiles", f"hbm_traffic_cfg{cfg}.txt")
-    roof["traffic"] = read_traffic(tfile, dominant)
+    # the committed summaries are of FP64-mode steps: another factor mode
+    # takes its traffic only from an explicit --traffic-file
+    tfile = args.traffic_file or (os.path.join(ROOT, "profiles", f"hbm_traffic_cfg{cfg}.txt")
+                                  if args.chol == "fp64" else None)
+    roof["traffic"] = read_traffic(tfile, dominant) if tfile else None
 
     # end to end: the host draws each batch's noise (MT19937 + polar normals,
     # main.cpp:479-486) on a thread while the GPU runs the previous batch
@@ -300,6 +315,8 @@ def main():
 
     # CPU baseline: the reference's own func.cpp composed into one step
     cpu = None
+    if args.cpu_steps is None:
+        args.cpu_steps = 0 if args.config in (3, 5) else 2
     if args.cpu_steps > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O

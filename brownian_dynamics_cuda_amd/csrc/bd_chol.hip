@@ -988,23 +988,16 @@ __global__ void __launch_bounds__(256, 2) k_trail(double *A, int ld, int k0, int
     store_tile(A, ld, ib * TB, jb * TB, As, diag);
 }
 
-// Three-slice BF16 trailing update with the panel tiles split once.  The
-// on-the-fly form above (k_trail<2>) splits every operand fragment in every
-// workgroup, so each panel tile is split by all the workgroups of its tile
+// Three-slice BF16 trailing update with the panel tiles split once.  Split
+// on the fly (k_trail<2>), every workgroup converts every operand fragment it
+// multiplies, so each panel tile is split by all the workgroups of its tile
 // row and column and the update is bound by that VALU work.  Here k_split3
 // writes the three BF16 slices of every parked panel tile of the super-panel
 // once, in the operand order of v_mfma_f32_16x16x32_bf16: fragment (rb, ks)
 // of a slice (16 rows from 16 rb, 32 K from 32 ks) is 64 lanes x 16 bytes,
 // lane l holding row 16 rb + l % 16 and K entries 32 ks + 8 (l / 16) .. +7.
 // Slice s of the tile of row block i and panel column t sits at
-// sp[((i - j0) nb + t) 3 + s].  k_trail_bf copies a panel column's 48
-// fragments (A and B operands, three slices) straight from global memory
-// into LDS (global_load_lds_dwordx4: lane l's 16 bytes land at the fragment's
-// slot + 16 l, so a fragment is one contiguous kilobyte in LDS as in global
-// memory and every wavefront read of it is conflict-free), each wavefront
-// reads its fragments into registers, and the next column's copy runs while
-// they are multiplied.  The six products, their order, the lane mapping and
-// the split are those of k_trail<2>: the same factor bit for bit.
+// sp[((i - j0) nb + t) 3 + s].
 constexpr size_t kSplitTile = (size_t)TB * TB;
 __device__ __forceinline__ size_t frag_off(int rb, int ks, int lane) { return (size_t)((rb * 2 + ks) * 64 + lane) * 8; }
 
@@ -1034,108 +1027,154 @@ __global__ void __launch_bounds__(256) k_split3(const double *A, int ld, int k0,
     }
 }
 
-// The LDS stage: 48 fragment slots of a kilobyte, slot (op, s, rb, ks) for
-// operand op (0: the row block's tiles, 1: the column block's; a diagonal
-// tile loads only op 0), slice s, row block rb, K step ks.  Wavefront w
-// copies slots w NLD .. w NLD + NLD - 1.
+// k_trail_bf: a workgroup takes a 2x2 block of trailing tiles (128x128) and
+// wavefront (wr, wc) the whole tile (ib0 + wr, jb0 + wc).  A stage is one K
+// step (32) of a panel column: the block's two row tiles' and two column
+// tiles' fragments, three slices each, 48 kilobyte slots (op, tile, s, rb),
+// copied straight from global memory into LDS (global_load_lds_dwordx4:
+// lane l's 16 bytes land at the slot + 16 l, so a fragment lies in LDS as in
+// global memory and every wavefront read of it is conflict-free).  The
+// copies are what bounds this kernel, so the tile is as large as the
+// registers allow (a wavefront reads 24 fragments per stage for 96
+// products), and three workgroups share a CU (3 x 48 KB of LDS) to keep
+// three stages in flight.  The six products of a 16x16 block, their order
+// and the split are those of k_trail<2>: the same factor bit for bit.
 typedef __attribute__((address_space(3))) void lds_void;
-__device__ __forceinline__ int stage_frag(int op, int s, int rb, int ks) { return (op * 3 + s) * 8 + rb * 2 + ks; }
-template <int NLD>
-__device__ __forceinline__ void issue_stage(const __bf16 *pa, const __bf16 *pb, int t, __bf16 *S, int w, int lane) {
+__device__ __forceinline__ int bf_slot(int op, int tile, int s, int rb) { return ((op * 2 + tile) * 3 + s) * 4 + rb; }
+__device__ __forceinline__ const __bf16 *bf_src(const __bf16 *sp, int row, int nb, int t, int s, int rb, int ks,
+                                                int lane) {
+    return sp + ((size_t)(row * nb + t) * 3 + s) * kSplitTile + frag_off(rb, ks, lane);
+}
+// Stage st (panel column st / 2, K step st % 2) of tile rows ra, ra + 1 and
+// (unless DIAG, where the column tiles are the row tiles) rbw, rbw + 1,
+// relative to j0; rows at or past nrow (the odd edge) are not copied.
+// Wavefront w copies one tile's 12 fragments (DIAG: half of one tile's).
+template <bool DIAG>
+__device__ __forceinline__ void bf_issue(const __bf16 *sp, int ra, int rbw, int nb, int nrow, int st, __bf16 *S, int w,
+                                         int lane) {
+    const int t = st >> 1, ks = st & 1;
+    if (DIAG) {
+        const int tile = w >> 1, half = w & 1;
+        if (ra + tile >= nrow) return;
 #pragma unroll
-    for (int j = 0; j < NLD; j++) {
-        const int f = w * NLD + j, op = f / 24, s = (f / 8) % 3, fr = f % 8;
-        const __bf16 *src = (op ? pb : pa) + (size_t)t * 3 * kSplitTile + s * kSplitTile + (size_t)fr * 512;
-        __builtin_amdgcn_global_load_lds(src + lane * 8, (lds_void *)(S + (size_t)f * 512), 16, 0, 0);
+        for (int j = 0; j < 6; j++) {
+            const int f = 6 * half + j, sl = f >> 2, rb = f & 3;
+            __builtin_amdgcn_global_load_lds(bf_src(sp, ra + tile, nb, t, sl, rb, ks, lane),
+                                             (lds_void *)(S + (size_t)bf_slot(0, tile, sl, rb) * 512), 16, 0, 0);
+        }
+    } else {
+        const int op = w >> 1, tile = w & 1, row = (op ? rbw : ra) + tile;
+        if (row >= nrow) return;
+#pragma unroll
+        for (int f = 0; f < 12; f++) {
+            const int sl = f >> 2, rb = f & 3;
+            __builtin_amdgcn_global_load_lds(bf_src(sp, row, nb, t, sl, rb, ks, lane),
+                                             (lds_void *)(S + (size_t)bf_slot(op, tile, sl, rb) * 512), 16, 0, 0);
+        }
     }
 }
-template <int BOP>
-__device__ __forceinline__ void stage_read(const __bf16 *S, int wr, int wc, int lane, bf8 (&fa)[2][3][2], bf8 (&fb)[2][3][2]) {
+// the six products of column block ni for the four row blocks, each
+// block's in the fixed order (smallest first)
+__device__ __forceinline__ void bf_mma_col(const bf8 (&fa)[3][4], const bf8 (&fb)[3], flt4 (&acc)[4][4], int ni) {
+    constexpr int PA[6] = {2, 1, 0, 1, 0, 0}, PB[6] = {0, 1, 2, 0, 1, 0};
 #pragma unroll
-    for (int ks = 0; ks < 2; ks++)
+    for (int p = 0; p < 6; p++)
 #pragma unroll
-        for (int q = 0; q < 2; q++)
-#pragma unroll
-            for (int s = 0; s < 3; s++) {
-                fa[ks][s][q] = *reinterpret_cast<const bf8 *>(S + (size_t)stage_frag(0, s, wr / 16 + q, ks) * 512 + lane * 8);
-                fb[ks][s][q] = *reinterpret_cast<const bf8 *>(S + (size_t)stage_frag(BOP, s, wc / 16 + q, ks) * 512 + lane * 8);
-            }
+        for (int mi = 0; mi < 4; mi++)
+            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[PA[p]][mi], fb[PB[p]], acc[mi][ni], 0, 0, 0);
 }
-__device__ __forceinline__ void frags_mma(const bf8 (&fa)[2][3][2], const bf8 (&fb)[2][3][2], flt4 (&acc)[2][2]) {
+__device__ __forceinline__ void bf_read_col(const __bf16 *S, int op, int tile, int q, int lane, bf8 (&fb)[3]) {
 #pragma unroll
-    for (int ks = 0; ks < 2; ks++)
-#pragma unroll
-        for (int mi = 0; mi < 2; mi++)
-#pragma unroll
-            for (int ni = 0; ni < 2; ni++) {
-                flt4 c = acc[mi][ni];
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ks][2][mi], fb[ks][0][ni], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ks][1][mi], fb[ks][1][ni], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ks][0][mi], fb[ks][2][ni], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ks][1][mi], fb[ks][0][ni], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ks][0][mi], fb[ks][1][ni], c, 0, 0, 0);
-                acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ks][0][mi], fb[ks][0][ni], c, 0, 0, 0);
-            }
+    for (int sl = 0; sl < 3; sl++)
+        fb[sl] = *reinterpret_cast<const bf8 *>(S + (size_t)bf_slot(op, tile, sl, q) * 512 + lane * 8);
 }
-// Per panel column t: every wavefront waits for its own copies (vmcnt(0);
-// the LDS writes of a copy are done when it retires), a barrier makes the
-// stage whole, each wavefront reads its 24 fragments into registers
-// (lgkmcnt(0)), a second barrier frees the stage, and the copy of column
-// t + 1 then runs beside the 48 products of column t.  The compiler does not
-// order the copies against these LDS reads, hence the explicit waits.
+// Per stage: every wavefront waits for its own copies (vmcnt(0); the LDS
+// writes of a copy are done when it retires) and a barrier makes the stage
+// whole; the row fragments (12) and two column blocks' (3 each) are read,
+// the products of column block 0 run while block 2's fragments are read into
+// block 0's registers (1 and 3 likewise); after lgkmcnt(0) a second barrier
+// frees the stage, the copy of the next stage is issued, and the products
+// of column blocks 2 and 3 run beside it.  The compiler does not order the
+// copies against these LDS reads, hence the explicit waits.  Wavefronts
+// with no tile (above the diagonal, past the edge) still copy and meet
+// every barrier.
 template <bool DIAG>
-__device__ __forceinline__ void trail_bf_body(const __bf16 *pa, const __bf16 *pb, int nb, __bf16 *St, int w, int lane,
-                                              int wr, int wc, flt4 (&acc)[2][2]) {
-    constexpr int NLD = DIAG ? 6 : 12, BOP = DIAG ? 0 : 1;
-    bf8 fa[2][3][2], fb[2][3][2];
-    issue_stage<NLD>(pa, pb, 0, St, w, lane);
-    for (int t = 0; t < nb; t++) {
+__device__ __forceinline__ void bf_body(const __bf16 *sp, int ra, int rbw, int nb, int nrow, __bf16 *S, int w, int lane,
+                                        bool active, flt4 (&acc)[4][4]) {
+    constexpr int BOP = DIAG ? 0 : 1;
+    const int wr = w >> 1, wc = w & 1;
+    bf8 fa[3][4], fb0[3], fb1[3];
+    bf_issue<DIAG>(sp, ra, rbw, nb, nrow, 0, S, w, lane);
+    for (int st = 0; st < 2 * nb; st++) {
         __builtin_amdgcn_s_waitcnt(0xF70);   // vmcnt(0)
         __builtin_amdgcn_s_barrier();
-        stage_read<BOP>(St, wr, wc, lane, fa, fb);
+        if (active) {
+#pragma unroll
+            for (int sl = 0; sl < 3; sl++)
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    fa[sl][q] = *reinterpret_cast<const bf8 *>(S + (size_t)bf_slot(0, wr, sl, q) * 512 + lane * 8);
+            bf_read_col(S, BOP, wc, 0, lane, fb0);
+            bf_read_col(S, BOP, wc, 1, lane, fb1);
+            bf_mma_col(fa, fb0, acc, 0);
+            bf_read_col(S, BOP, wc, 2, lane, fb0);
+            bf_mma_col(fa, fb1, acc, 1);
+            bf_read_col(S, BOP, wc, 3, lane, fb1);
+        }
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
         __builtin_amdgcn_s_barrier();
-        if (t + 1 < nb) issue_stage<NLD>(pa, pb, t + 1, St, w, lane);
-        frags_mma(fa, fb, acc);
+        if (st + 1 < 2 * nb) bf_issue<DIAG>(sp, ra, rbw, nb, nrow, st + 1, S, w, lane);
+        if (active) {
+            bf_mma_col(fa, fb0, acc, 2);
+            bf_mma_col(fa, fb1, acc, 3);
+        }
     }
 }
-__global__ void __launch_bounds__(256, 2) k_trail_bf(double *A, int ld, const __bf16 *sp, int nb, int j0, int nt) {
-    __shared__ __attribute__((aligned(16))) __bf16 St[48 * 512];
-    int SI, SJ;
-    tri_index(blockIdx.x >> 6, SI, SJ);
-    const int ib = j0 + 8 * SI + ((blockIdx.x >> 3) & 7), jb = j0 + 8 * SJ + (blockIdx.x & 7);
-    if (ib >= nt || jb > ib) return;
-    const bool diag = ib == jb;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int wr = (w >> 1) * 32, wc = (w & 1) * 32, i16 = lane & 15, k4 = lane >> 4;
-    const __bf16 *pa = sp + (size_t)(ib - j0) * nb * 3 * kSplitTile;
-    const __bf16 *pb = sp + (size_t)(jb - j0) * nb * 3 * kSplitTile;
-    flt4 acc[2][2];
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) acc[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
-    if (diag) trail_bf_body<true>(pa, pb, nb, St, w, lane, wr, wc, acc);
-    else trail_bf_body<false>(pa, pb, nb, St, w, lane, wr, wc, acc);
-    // C -= acc in place: all sixteen loads first, then the stores
-    double cv[2][2][4];
+// Workgroups go round-robin over the eight XCDs (b % 8); XCD x takes groups
+// x, x + 8, ... of 8x8 blocks, 64 consecutive workgroups of its own per
+// group, so a group's 16 row and 16 column panel tiles are read into one L2.
+__global__ void __launch_bounds__(256, 3) k_trail_bf(double *A, int ld, const __bf16 *sp, int nb, int j0, int nt) {
+    __shared__ __attribute__((aligned(16))) __bf16 S[48 * 512];
+    const int S2 = (nt - j0 + 1) / 2, SG = (S2 + 7) / 8, l = blockIdx.x >> 3, G = (l >> 6) * 8 + (blockIdx.x & 7);
+    if (G >= SG * (SG + 1) / 2) return;
+    int GI, GJ;
+    tri_index(G, GI, GJ);
+    const int IB = 8 * GI + ((l >> 3) & 7), JB = 8 * GJ + (l & 7);
+    if (IB >= S2 || JB > IB) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
+    const int nrow = nt - j0, ra = 2 * IB, rbw = 2 * JB;  // tile rows relative to j0
+    const int ib = j0 + ra + wr, jb = j0 + rbw + wc;
+    const bool active = ib < nt && jb <= ib;
+    flt4 acc[4][4];
 #pragma unroll
-    for (int mi = 0; mi < 2; mi++)
+    for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int ni = 0; ni < 2; ni++)
+        for (int b = 0; b < 4; b++) acc[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
+    if (IB == JB) bf_body<true>(sp, ra, rbw, nb, nrow, S, w, lane, active, acc);
+    else bf_body<false>(sp, ra, rbw, nb, nrow, S, w, lane, active, acc);
+    if (!active) return;
+    const int i16 = lane & 15, k4 = lane >> 4;
+    // C -= acc in place, a row of 16x16 blocks at a time: loads, then stores
+    // (a diagonal tile keeps its upper triangle)
+#pragma unroll
+    for (int mi = 0; mi < 4; mi++) {
+        double cv[4][4];
+#pragma unroll
+        for (int ni = 0; ni < 4; ni++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int r = wr + 16 * mi + 4 * k4 + v, c = wc + 16 * ni + i16;
-                cv[mi][ni][v] = A[(size_t)(ib * TB + r) * ld + jb * TB + c];
+                const int r = 16 * mi + 4 * k4 + v, c = 16 * ni + i16;
+                cv[ni][v] = A[(size_t)(ib * TB + r) * ld + jb * TB + c];
             }
 #pragma unroll
-    for (int mi = 0; mi < 2; mi++)
-#pragma unroll
-        for (int ni = 0; ni < 2; ni++)
+        for (int ni = 0; ni < 4; ni++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int r = wr + 16 * mi + 4 * k4 + v, c = wc + 16 * ni + i16;
-                if (diag && c > r) continue;
-                A[(size_t)(ib * TB + r) * ld + jb * TB + c] = cv[mi][ni][v] + -1.0 * (double)acc[mi][ni][v];
+                const int r = 16 * mi + 4 * k4 + v, c = 16 * ni + i16;
+                if (ib == jb && c > r) continue;
+                A[(size_t)(ib * TB + r) * ld + jb * TB + c] = cv[ni][v] + -1.0 * (double)acc[mi][ni][v];
             }
+    }
 }
 
 // Factor and inverse of the diagonal tile (d, d) by one workgroup (the
@@ -1346,7 +1385,8 @@ void chol_trail(const CholWork &c, int k0, int nb, hipStream_t q) {
     if (MODE == 2 && c.sp && nb <= c.sp_nb) {
         __bf16 *sp = static_cast<__bf16 *>(c.sp);
         k_split3<<<(nt - j0) * nb, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, sp);
-        k_trail_bf<<<SM * (SM + 1) / 2 * 64, 256, 0, q>>>(c.A, c.ld, sp, nb, j0, nt);
+        const int S2 = (nt - j0 + 1) / 2, SG = (S2 + 7) / 8;  // 128x128 blocks, 8x8 groups
+        k_trail_bf<<<(SG * (SG + 1) / 2 + 7) / 8 * 512, 256, 0, q>>>(c.A, c.ld, sp, nb, j0, nt);
         return;
     }
     k_trail<MODE><<<SM * (SM + 1) / 2 * 64, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, nt);
