@@ -174,8 +174,8 @@ def main():
     trail_fl = []
     k0 = 0
     while nb and k0 + nb < nt:
-        ms = n_hi3 - 64 * (k0 + nb)
-        trail_fl.append(64.0 * nb * ms * (ms + 1))
+        mrows = n_hi3 - 64 * (k0 + nb)
+        trail_fl.append(64.0 * nb * mrows * (mrows + 1))
         k0 += nb
     if dominant in ("chol_step", "potrf"):
         # the factorisation's algorithmic FLOPs less the trailing updates',
