@@ -189,12 +189,19 @@ __device__ __forceinline__ void store_tile(double *A, int ld, int r0, int c0, co
 __device__ __forceinline__ void mma_f64(const double *As, const double *Bs, dbl4 acc[2][2]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wr = (w >> 1) * 32, wc = (w & 1) * 32, i16 = lane & 15, k4 = lane >> 4;
-#pragma unroll 4
+    // the operands of K step kk + 4 are read while the products of step kk run
+    const double *pa0 = As + (wr + i16) * LDP + k4, *pa1 = pa0 + 16 * LDP;
+    const double *pb0 = Bs + (wc + i16) * LDP + k4, *pb1 = pb0 + 16 * LDP;
+    double na0 = pa0[0], na1 = pa1[0], nb0 = pb0[0], nb1 = pb1[0];
+#pragma unroll
     for (int kk = 0; kk < TB; kk += 4) {
-        double a0 = As[(wr + i16) * LDP + kk + k4];
-        double a1 = As[(wr + 16 + i16) * LDP + kk + k4];
-        double b0 = Bs[(wc + i16) * LDP + kk + k4];
-        double b1 = Bs[(wc + 16 + i16) * LDP + kk + k4];
+        const double a0 = na0, a1 = na1, b0 = nb0, b1 = nb1;
+        if (kk + 4 < TB) {
+            na0 = pa0[kk + 4];
+            na1 = pa1[kk + 4];
+            nb0 = pb0[kk + 4];
+            nb1 = pb1[kk + 4];
+        }
         acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
         acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
         acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
@@ -207,12 +214,18 @@ __device__ __forceinline__ void mma_f64(const double *As, const double *Bs, dbl4
 __device__ __forceinline__ void mma_f32(const double *As, const double *Bs, flt4 acc[2][2]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wr = (w >> 1) * 32, wc = (w & 1) * 32, i16 = lane & 15, k4 = lane >> 4;
-#pragma unroll 4
+    const double *pa0 = As + (wr + i16) * LDP + k4, *pa1 = pa0 + 16 * LDP;
+    const double *pb0 = Bs + (wc + i16) * LDP + k4, *pb1 = pb0 + 16 * LDP;
+    double na0 = pa0[0], na1 = pa1[0], nb0 = pb0[0], nb1 = pb1[0];
+#pragma unroll
     for (int kk = 0; kk < TB; kk += 4) {
-        float a0 = (float)As[(wr + i16) * LDP + kk + k4];
-        float a1 = (float)As[(wr + 16 + i16) * LDP + kk + k4];
-        float b0 = (float)Bs[(wc + i16) * LDP + kk + k4];
-        float b1 = (float)Bs[(wc + 16 + i16) * LDP + kk + k4];
+        const float a0 = (float)na0, a1 = (float)na1, b0 = (float)nb0, b1 = (float)nb1;
+        if (kk + 4 < TB) {
+            na0 = pa0[kk + 4];
+            na1 = pa1[kk + 4];
+            nb0 = pb0[kk + 4];
+            nb1 = pb1[kk + 4];
+        }
         acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
         acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
         acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
@@ -321,18 +334,8 @@ __device__ __forceinline__ void load_tiles(int n, const double *const (&src)[4],
             }
 }
 
-// 16x16 output blocks on the FP64 matrix cores by one wavefront (mapping of
-// mma_f64: register v of lane l is element (l/16 + 4v, l%16) of the block).
-// acc += A[16I.., 0:K) B[16J.., 0:K)^T
-__device__ __forceinline__ dbl4 blk_mma(const double *A, const double *B, int I, int J, int K, dbl4 acc) {
-    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
-    for (int kk = 0; kk < K; kk += 4) {
-        const double a = A[(16 * I + i16) * LDP + kk + k4];
-        const double b = B[(16 * J + i16) * LDP + kk + k4];
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-    }
-    return acc;
-}
+// A 16x16 output block of the FP64 matrix cores (mapping of mma_f64:
+// register v of lane l is element (l/16 + 4v, l%16) of the block) into C.
 __device__ __forceinline__ void blk_to_lds(double *C, int I, int J, dbl4 acc, double sign, bool overwrite) {
     const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
 #pragma unroll
@@ -350,15 +353,25 @@ __device__ __forceinline__ void blk_to_lds(double *C, int I, int J, dbl4 acc, do
 // nothing to any sum, so P is the full product bit for bit.
 __device__ __forceinline__ void mma_panel(const double *As, const double *Li, dbl4 (&acc)[4]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i16 = lane & 15, k4 = lane >> 4;
+    const double *pa = As + (16 * w + i16) * LDP + k4, *pl = Li + i16 * LDP + k4;
+    double na = pa[0], nb[4];
+#pragma unroll
+    for (int J = 0; J < 4; J++) nb[J] = pl[16 * J * LDP];
 #pragma unroll
     for (int kk = 0; kk < TB; kk += 4) {
-        const double a = As[(16 * w + i16) * LDP + kk + k4];
+        const double a = na;
+        double b[4];
+#pragma unroll
+        for (int J = 0; J < 4; J++) b[J] = nb[J];
+        if (kk + 4 < TB) {  // next K step's operands beside this step's products
+            na = pa[kk + 4];
+#pragma unroll
+            for (int J = 0; J < 4; J++)
+                if (kk + 4 < 16 * (J + 1)) nb[J] = pl[16 * J * LDP + kk + 4];
+        }
 #pragma unroll
         for (int J = 0; J < 4; J++)
-            if (kk < 16 * (J + 1)) {
-                const double b = Li[(16 * J + i16) * LDP + kk + k4];
-                acc[J] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[J], 0, 0, 0);
-            }
+            if (kk < 16 * (J + 1)) acc[J] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[J], acc[J], 0, 0, 0);
     }
 }
 __device__ __forceinline__ void panel_to_lds(double *C, const dbl4 (&acc)[4]) {
@@ -368,27 +381,50 @@ __device__ __forceinline__ void panel_to_lds(double *C, const dbl4 (&acc)[4]) {
 }
 
 // Diagonal tile update C -= P P^T on its ten lower 16x16 blocks only (the
-// factor never reads above the diagonal), three or two per wavefront.
+// factor never reads above the diagonal), three or two per wavefront, their
+// products interleaved (independent accumulators) with the next K step's
+// operands read beside them; each block sums K in increasing order from
+// zero and is then subtracted from C.
+template <int N, int I0, int J0, int I1, int J1, int I2, int J2>
+__device__ __forceinline__ void syrk_blocks(const double *P, double *C) {
+    constexpr int I[3] = {I0, I1, I2}, J[3] = {J0, J1, J2};
+    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
+    const double *p = P + i16 * LDP + k4;
+    dbl4 acc[N];
+    double na[N], nb[N];
+#pragma unroll
+    for (int b = 0; b < N; b++) {
+        acc[b] = dbl4{0.0, 0.0, 0.0, 0.0};
+        na[b] = p[16 * I[b] * LDP];
+        nb[b] = p[16 * J[b] * LDP];
+    }
+#pragma unroll
+    for (int kk = 0; kk < TB; kk += 4) {
+        double a[N], bv[N];
+#pragma unroll
+        for (int b = 0; b < N; b++) {
+            a[b] = na[b];
+            bv[b] = nb[b];
+        }
+        if (kk + 4 < TB)
+#pragma unroll
+            for (int b = 0; b < N; b++) {
+                na[b] = p[16 * I[b] * LDP + kk + 4];
+                nb[b] = p[16 * J[b] * LDP + kk + 4];
+            }
+#pragma unroll
+        for (int b = 0; b < N; b++) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[b], bv[b], acc[b], 0, 0, 0);
+    }
+#pragma unroll
+    for (int b = 0; b < N; b++) blk_to_lds(C, I[b], J[b], acc[b], -1.0, false);
+}
 __device__ __forceinline__ void syrk_lower(const double *P, double *C) {
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // (I, J) blocks of wavefront w: {00 10 11} {20 21 22} {30 31} {32 33}
-    const int I0 = w < 2 ? 2 * w : 3, J0 = w < 2 ? 0 : 2 * (w - 2);
-    const dbl4 z = {0.0, 0.0, 0.0, 0.0};
-    if (w == 0) {
-        const dbl4 c0 = blk_mma(P, P, 0, 0, TB, z), c1 = blk_mma(P, P, 1, 0, TB, z), c2 = blk_mma(P, P, 1, 1, TB, z);
-        blk_to_lds(C, 0, 0, c0, -1.0, false);
-        blk_to_lds(C, 1, 0, c1, -1.0, false);
-        blk_to_lds(C, 1, 1, c2, -1.0, false);
-    } else if (w == 1) {
-        const dbl4 c0 = blk_mma(P, P, 2, 0, TB, z), c1 = blk_mma(P, P, 2, 1, TB, z), c2 = blk_mma(P, P, 2, 2, TB, z);
-        blk_to_lds(C, 2, 0, c0, -1.0, false);
-        blk_to_lds(C, 2, 1, c1, -1.0, false);
-        blk_to_lds(C, 2, 2, c2, -1.0, false);
-    } else {
-        const dbl4 c0 = blk_mma(P, P, I0, J0, TB, z), c1 = blk_mma(P, P, I0, J0 + 1, TB, z);
-        blk_to_lds(C, I0, J0, c0, -1.0, false);
-        blk_to_lds(C, I0, J0 + 1, c1, -1.0, false);
-    }
+    if (w == 0) syrk_blocks<3, 0, 0, 1, 0, 1, 1>(P, C);
+    else if (w == 1) syrk_blocks<3, 2, 0, 2, 1, 2, 2>(P, C);
+    else if (w == 2) syrk_blocks<2, 3, 0, 3, 1, 0, 0>(P, C);
+    else syrk_blocks<2, 3, 2, 3, 3, 0, 0>(P, C);
 }
 
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
