@@ -1213,6 +1213,163 @@ __global__ void __launch_bounds__(256, 3) k_trail_bf(double *A, int ld, const __
     }
 }
 
+// FP64 / FP32 trailing update of a super-panel from panel tiles packed once.
+// k_pack64 copies each parked panel tile of the super-panel into the operand
+// order of v_mfma_f64_16x16x4f64 (v_mfma_f32_16x16x4f32 has the same lane
+// layout): unit (rb, kp) of a tile is 64 lanes x 16 bytes, lane l holding
+// row 16 rb + l % 16 at K = 8 kp + l / 16 and 8 kp + 4 + l / 16, the
+// operands of K steps 2 kp and 2 kp + 1.  The tile of row block i and panel
+// column t sits at pk[((i - j0) nb + t) TB TB].
+constexpr size_t kPackTile = (size_t)TB * TB;  // doubles
+__device__ __forceinline__ size_t pk_off(int rb, int kp, int lane) { return (size_t)((rb * 8 + kp) * 64 + lane) * 2; }
+
+// one workgroup per (row block, panel column) tile; wavefront w writes the
+// units of row block w
+__global__ void __launch_bounds__(256) k_pack64(const double *A, int ld, int k0, int nb, int j0, double *pk) {
+    const int i = j0 + blockIdx.x / nb, t = blockIdx.x % nb;
+    const double *src = A + (size_t)(k0 + t) * TB * ld + (size_t)i * TB;
+    double *dst = pk + (size_t)blockIdx.x * kPackTile;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = 16 * w + (lane & 15), k4 = lane >> 4;
+#pragma unroll
+    for (int kp = 0; kp < 8; kp++) {
+        dbl2 v;
+        v.x = src[(size_t)r * ld + 8 * kp + k4];
+        v.y = src[(size_t)r * ld + 8 * kp + 4 + k4];
+        *reinterpret_cast<dbl2 *>(dst + pk_off(w, kp, lane)) = v;
+    }
+}
+
+// k_trail_pk: a workgroup takes a 2x2 block of trailing tiles (128x128),
+// wavefront (wr, wc) the whole tile (ib0 + wr, jb0 + wc), as k_trail_bf
+// does.  A stage is 16 of a panel column's K: units (rb, kp) of the
+// block's two row and two column tiles, 32 KB in slots (op, tile, rb, kp),
+// copied straight from global memory into LDS.  Two stages per workgroup:
+// the copy of stage s + 1 is issued before the products of stage s, so it
+// has a whole stage to land; two workgroups per CU (2 x 64 KB of LDS).
+// Every accumulator takes the MFMAs of k_trail<MODE> in the same order (panel
+// column, then K step), so the update is the same bit for bit.
+__device__ __forceinline__ int pk_slot(int op, int tile, int rb, int kpl) { return ((op * 2 + tile) * 4 + rb) * 2 + kpl; }
+template <bool DIAG>
+__device__ __forceinline__ void pk_issue(const double *pk, int ra, int rbw, int nb, int nrow, int st, double *S, int w,
+                                         int lane) {
+    const int t = st >> 2, h = st & 3;
+    if (DIAG) {
+        const int tile = w >> 1, half = w & 1;
+        if (ra + tile >= nrow) return;
+        const double *src = pk + (size_t)((ra + tile) * nb + t) * kPackTile;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int rb = 2 * half + (j >> 1), kpl = j & 1;
+            __builtin_amdgcn_global_load_lds(src + pk_off(rb, 2 * h + kpl, lane),
+                                             (lds_void *)(S + (size_t)pk_slot(0, tile, rb, kpl) * 128), 16, 0, 0);
+        }
+    } else {
+        const int op = w >> 1, tile = w & 1, row = (op ? rbw : ra) + tile;
+        if (row >= nrow) return;
+        const double *src = pk + (size_t)(row * nb + t) * kPackTile;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int rb = j >> 1, kpl = j & 1;
+            __builtin_amdgcn_global_load_lds(src + pk_off(rb, 2 * h + kpl, lane),
+                                             (lds_void *)(S + (size_t)pk_slot(op, tile, rb, kpl) * 128), 16, 0, 0);
+        }
+    }
+}
+template <int MODE, typename V>
+__device__ __forceinline__ V pk_mfma(double a, double b, V c) {
+    if constexpr (MODE == 0) return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_16x16x4f32((float)a, (float)b, c, 0, 0, 0);
+}
+template <int MODE, bool DIAG, typename V>
+__device__ __forceinline__ void pk_body(const double *pk, int ra, int rbw, int nb, int nrow, double *S, int w, int lane,
+                                        bool active, V (&acc)[4][4]) {
+    constexpr int BOP = DIAG ? 0 : 1;
+    const int wr = w >> 1, wc = w & 1, ns = 4 * nb;
+    pk_issue<DIAG>(pk, ra, rbw, nb, nrow, 0, S, w, lane);
+    for (int st = 0; st < ns; st++) {
+        const double *B = S + (size_t)(st & 1) * 32 * 128;
+        if (st + 1 < ns) {
+            pk_issue<DIAG>(pk, ra, rbw, nb, nrow, st + 1, S + (size_t)((st + 1) & 1) * 32 * 128, w, lane);
+            // this wavefront's copies of stage st are done, those of st + 1
+            // may be in flight (vector memory counts retire in order)
+            if (DIAG) __builtin_amdgcn_s_waitcnt(0xF74);  // vmcnt(4)
+            else __builtin_amdgcn_s_waitcnt(0xF78);       // vmcnt(8)
+        } else {
+            __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
+        }
+        __builtin_amdgcn_s_barrier();
+        if (active) {
+            dbl2 a[2][4], b[2][4];
+#pragma unroll
+            for (int kpl = 0; kpl < 2; kpl++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    a[kpl][q] = *reinterpret_cast<const dbl2 *>(B + (size_t)pk_slot(0, wr, q, kpl) * 128 + lane * 2);
+                    b[kpl][q] = *reinterpret_cast<const dbl2 *>(B + (size_t)pk_slot(BOP, wc, q, kpl) * 128 + lane * 2);
+                }
+#pragma unroll
+            for (int kpl = 0; kpl < 2; kpl++) {
+#pragma unroll
+                for (int mi = 0; mi < 4; mi++)
+#pragma unroll
+                    for (int ni = 0; ni < 4; ni++) acc[mi][ni] = pk_mfma<MODE>(a[kpl][mi].x, b[kpl][ni].x, acc[mi][ni]);
+#pragma unroll
+                for (int mi = 0; mi < 4; mi++)
+#pragma unroll
+                    for (int ni = 0; ni < 4; ni++) acc[mi][ni] = pk_mfma<MODE>(a[kpl][mi].y, b[kpl][ni].y, acc[mi][ni]);
+            }
+        }
+        // every wavefront has read stage st before its slots take stage st + 2
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+        __builtin_amdgcn_s_barrier();
+    }
+}
+template <int MODE>
+__global__ void __launch_bounds__(256, 2) k_trail_pk(double *A, int ld, const double *pk, int nb, int j0, int nt) {
+    __shared__ __attribute__((aligned(16))) double S[2 * 32 * 128];
+    const int S2 = (nt - j0 + 1) / 2, SG = (S2 + 7) / 8, l = blockIdx.x >> 3, G = (l >> 6) * 8 + (blockIdx.x & 7);
+    if (G >= SG * (SG + 1) / 2) return;
+    int GI, GJ;
+    tri_index(G, GI, GJ);
+    const int IB = 8 * GI + ((l >> 3) & 7), JB = 8 * GJ + (l & 7);
+    if (IB >= S2 || JB > IB) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
+    const int nrow = nt - j0, ra = 2 * IB, rbw = 2 * JB;  // tile rows relative to j0
+    const int ib = j0 + ra + wr, jb = j0 + rbw + wc;
+    const bool active = ib < nt && jb <= ib;
+    typedef typename AccOf<MODE>::T V;
+    V acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = V{};
+    if (IB == JB) pk_body<MODE, true>(pk, ra, rbw, nb, nrow, S, w, lane, active, acc);
+    else pk_body<MODE, false>(pk, ra, rbw, nb, nrow, S, w, lane, active, acc);
+    if (!active) return;
+    const int i16 = lane & 15, k4 = lane >> 4;
+    // C -= acc in place (row of register v: l/16 + 4v for the FP64 MFMA,
+    // 4(l/16) + v for the FP32 one), a row of 16x16 blocks at a time
+#pragma unroll
+    for (int mi = 0; mi < 4; mi++) {
+        double cv[4][4];
+#pragma unroll
+        for (int ni = 0; ni < 4; ni++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = 16 * mi + (MODE == 0 ? k4 + 4 * v : 4 * k4 + v), c = 16 * ni + i16;
+                cv[ni][v] = A[(size_t)(ib * TB + r) * ld + jb * TB + c];
+            }
+#pragma unroll
+        for (int ni = 0; ni < 4; ni++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = 16 * mi + (MODE == 0 ? k4 + 4 * v : 4 * k4 + v), c = 16 * ni + i16;
+                if (ib == jb && c > r) continue;
+                A[(size_t)(ib * TB + r) * ld + jb * TB + c] = cv[ni][v] + -1.0 * (double)acc[mi][ni][v];
+            }
+    }
+}
+
 // Factor and inverse of the diagonal tile (d, d) by one workgroup (the
 // tile after a super-panel's trailing update, or the first tile of a
 // caller's matrix); clear != 0 also restarts the negative-pivot count.
@@ -1417,19 +1574,35 @@ void chol_column(const CholWork &c, int kb, int jcols, hipStream_t q) {
 }
 template <int MODE>
 void chol_trail(const CholWork &c, int k0, int nb, hipStream_t q) {
-    const int nt = c.mpad / TB, j0 = k0 + nb, SM = (nt - j0 + 7) / 8;
-    if (MODE == 2 && c.sp && nb <= c.sp_nb) {
-        __bf16 *sp = static_cast<__bf16 *>(c.sp);
-        k_split3<<<(nt - j0) * nb, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, sp);
-        const int S2 = (nt - j0 + 1) / 2, SG = (S2 + 7) / 8;  // 128x128 blocks, 8x8 groups
-        k_trail_bf<<<(SG * (SG + 1) / 2 + 7) / 8 * 512, 256, 0, q>>>(c.A, c.ld, sp, nb, j0, nt);
+    const int nt = c.mpad / TB, j0 = k0 + nb;
+    // the 128x128 blocks pay in FP64 from 192 trailing block rows, in FP32
+    // from about 96 (fewer blocks than workgroup slots below that); in BF16
+    // always (the direct form splits every operand fragment in every
+    // workgroup)
+    const int min_rows = c.pack_min_rows >= 0 ? c.pack_min_rows : MODE == 0 ? 192 : MODE == 1 ? 96 : 0;
+    if (c.sp && nb <= c.sp_nb && nt - j0 >= min_rows) {
+        // panel tiles packed (split) once, 128x128 blocks in 8x8 groups
+        const int S2 = (nt - j0 + 1) / 2, SG = (S2 + 7) / 8, grid = (SG * (SG + 1) / 2 + 7) / 8 * 512;
+        if constexpr (MODE == 2) {
+            __bf16 *sp = static_cast<__bf16 *>(c.sp);
+            k_split3<<<(nt - j0) * nb, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, sp);
+            k_trail_bf<<<grid, 256, 0, q>>>(c.A, c.ld, sp, nb, j0, nt);
+        } else {
+            double *pk = static_cast<double *>(c.sp);
+            k_pack64<<<(nt - j0) * nb, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, pk);
+            k_trail_pk<MODE><<<grid, 256, 0, q>>>(c.A, c.ld, pk, nb, j0, nt);
+        }
         return;
     }
+    const int SM = (nt - j0 + 7) / 8;
     k_trail<MODE><<<SM * (SM + 1) / 2 * 64, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, nt);
 }
 }  // namespace
 
-size_t split_bytes(int ld, int nb) { return (size_t)(ld / TB) * nb * 3 * kSplitTile * sizeof(__bf16); }
+size_t split_bytes(int ld, int nb) {
+    return std::max((size_t)(ld / TB) * nb * 3 * kSplitTile * sizeof(__bf16),
+                    (size_t)(ld / TB) * nb * kPackTile * sizeof(double));
+}
 
 int chol_panel_tiles(const CholWork &c) {
     const int nt = c.mpad / TB;

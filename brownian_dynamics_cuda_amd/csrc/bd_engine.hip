@@ -552,7 +552,10 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     r |= c->alloc(&c->chol.xs, (size_t)c->chol.ld);
     r |= c->alloc(&c->chol.rv, 2 * (size_t)c->chol.ld);
     r |= c->alloc(&c->chol.pr, (size_t)c->chol.ld);
-    if (!getenv("BD_BF16_SPLIT") || atoi(getenv("BD_BF16_SPLIT")) != 0) {
+    // super-panel panel tiles packed (FP64 / FP32) or split (BF16) once for
+    // the trailing update; BD_PANEL_PACK=0 reads the parked tiles directly
+    c->chol.pack_min_rows = getenv("BD_PANEL_PACK_MIN_ROWS") ? atoi(getenv("BD_PANEL_PACK_MIN_ROWS")) : -1;
+    if (!getenv("BD_PANEL_PACK") || atoi(getenv("BD_PANEL_PACK")) != 0) {
         double *spd = nullptr;
         c->chol.sp_nb = 8;
         r |= c->alloc(&spd, (split_bytes(c->chol.ld, c->chol.sp_nb) + 7) / 8);

@@ -84,12 +84,18 @@ struct CholWork {
     // rv, two work vectors of ld doubles; pr, D inv(L~)^T p (ld)
     int refine;
     double *xs, *rv, *pr;
-    // BF16 mode: the super-panels' panel tiles split once into three BF16
-    // slices (split_bytes(ld, sp_nb) bytes; null: split on the fly)
+    // the super-panels' panel tiles packed once in matrix-core operand order
+    // (FP64 / FP32 modes) or split into three BF16 slices (BF16 mode);
+    // split_bytes(ld, sp_nb) bytes; null: the trailing update reads the
+    // parked tiles directly
     void *sp;
     int sp_nb;
+    // smallest trailing matrix (block rows) that takes the packed form; -1:
+    // the mode's default
+    int pack_min_rows;
 };
-// bytes of the BF16 split buffer for leading dimension ld and panels of nb
+// bytes of the packed / split panel buffer for leading dimension ld and
+// panels of nb
 size_t split_bytes(int ld, int nb);
 // super-panel width the factor of c uses (0: one block column at a time)
 int chol_panel_tiles(const CholWork &c);

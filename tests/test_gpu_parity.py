@@ -363,22 +363,29 @@ def test_super_panel_schedule_factor(panel):
         assert e <= 1e-6 and rel_err(Ls[mode], L0) <= 1e-5
 
 
+@pytest.mark.parametrize("mode", [E.CHOL_FP64, E.CHOL_FP32, E.CHOL_BF16])
 @pytest.mark.parametrize("panel", [2, 3, 8])
-def test_bf16_presplit_equals_on_the_fly(panel, monkeypatch):
-    """The BF16 trailing update from panel tiles split once per super-panel
-    (k_split3 + k_trail_bf, the default) against the form that splits every
-    operand fragment in every workgroup (k_trail<2>, BD_BF16_SPLIT=0): the
-    same products in the same order, so the same factor bit for bit (6-core
-    deck, full hydrodynamic set, D 1404 x 1404, 22 block columns)."""
+def test_packed_panels_equal_direct(mode, panel, monkeypatch):
+    """The super-panel trailing update from panel tiles packed once in
+    matrix-core operand order (k_pack64 + k_trail_pk in FP64 and FP32,
+    k_split3 + k_trail_bf in three-slice BF16; the default) against the form
+    that reads the parked tiles directly (k_trail<MODE>, BD_PANEL_PACK=0,
+    which splits every BF16 operand fragment in every workgroup): the same
+    products in the same order, so the same factor bit for bit (6-core deck,
+    full hydrodynamic set, D 1404 x 1404, 22 block columns; panels of 3 give
+    odd trailing edges)."""
     fib, st = deck("small", hi_all=1)
+    # the packed form for every super-panel (by default FP64 takes it from
+    # 192 trailing block rows, FP32 from 96)
+    monkeypatch.setenv("BD_PANEL_PACK_MIN_ROWS", "0")
     Ls = {}
-    for split in ("1", "0"):
-        monkeypatch.setenv("BD_BF16_SPLIT", split)
-        with E.BDEngine(fib, chol_mode=E.CHOL_BF16, use_graph=False) as eng:
+    for pack in ("1", "0"):
+        monkeypatch.setenv("BD_PANEL_PACK", pack)
+        with E.BDEngine(fib, chol_mode=mode, use_graph=False) as eng:
             eng.set_chol_schedule(panel)
             eng.set_state(st, rebuild=True)
             eng.factor()
-            Ls[split], neg = eng.get_factor()
+            Ls[pack], neg = eng.get_factor()
             assert neg == 0
     assert np.array_equal(Ls["1"], Ls["0"])
 
