@@ -1551,10 +1551,16 @@ void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, co
                 bool factor_first) {
     const int nt = c.mpad / TB;
     if (c.xs && c.m > 0) k_hi_snapshot<<<nblk(c.m / 3, 256), 256, 0, q>>>(s, g, c.m / 3, c.xs);
-    if (factor_first)
+    // the first tile's factor inside the assembly launch saves a launch on
+    // the factor's chain, but gives every workgroup the factor's LDS (one
+    // per CU): from 64 block columns the assembly takes its own launch
+    // (four workgroups per CU) and k_diag the first tile
+    if (factor_first && nt < 64) {
         k_rpy<true><<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2, c.neg, c.Linv);
-    else
+    } else {
         k_rpy<false><<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2, c.neg, c.Linv);
+        if (factor_first) launch_diag(c, 0, false, q);
+    }
     (void)n_hi;
 }
 
