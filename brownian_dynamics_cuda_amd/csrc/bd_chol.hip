@@ -21,9 +21,12 @@
 // and the trailing matrix then receives the whole panel in one launch
 // (k_trail: each tile accumulates K = 64 NB in registers before its one
 // read-modify-write), so the trailing matrix streams through HBM once per
-// NB block columns rather than once per block column.  In the FP32 and BF16
-// modes only that trailing update takes the reduced precision; the panel's
-// own columns stay FP64.
+// NB block columns rather than once per block column.  On large trailing
+// matrices the panel tiles are first copied once into the operand order of
+// the matrix cores (k_pack64; k_split3 splits them into BF16 slices) and
+// the update takes 128x128 blocks fed by global-to-LDS copies (k_trail_pk,
+// k_trail_bf).  In the FP32 and BF16 modes only that trailing update takes
+// the reduced precision; the panel's own columns stay FP64.
 #include <algorithm>
 
 #include "bd_dev.h"
