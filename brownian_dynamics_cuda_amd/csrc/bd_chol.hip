@@ -990,13 +990,26 @@ __device__ __forceinline__ void regs_to_lds(const dbl2 (&v)[TB * TB / 2 / NTHR_T
         S[r * LDP + c + 1] = v[i].y;
     }
 }
+// The target of a trailing-update launch is the lower tiles (i, j) with
+// jlo <= j < jhi, i from j to nt - 1: jhi == nt the whole triangle from jlo,
+// jhi < nt the strip of the next super-panel's columns, which is updated
+// ahead of the rest.
 template <int MODE>
-__global__ void __launch_bounds__(256, 2) k_trail(double *A, int ld, int k0, int nb, int j0, int nt) {
+__global__ void __launch_bounds__(256, 2) k_trail(double *A, int ld, int k0, int nb, int jlo, int jhi, int nt) {
     __shared__ double As[TB * LDP], Bs[TB * LDP];
-    int SI, SJ;
-    tri_index(blockIdx.x >> 6, SI, SJ);
-    const int ib = j0 + 8 * SI + ((blockIdx.x >> 3) & 7), jb = j0 + 8 * SJ + (blockIdx.x & 7);
-    if (ib >= nt || jb > ib) return;
+    int ib, jb;
+    if (jhi >= nt) {
+        int SI, SJ;
+        tri_index(blockIdx.x >> 6, SI, SJ);
+        ib = jlo + 8 * SI + ((blockIdx.x >> 3) & 7);
+        jb = jlo + 8 * SJ + (blockIdx.x & 7);
+        if (ib >= nt || jb > ib) return;
+    } else {
+        int ii, jj;
+        trap_index(blockIdx.x, jhi - jlo, ii, jj);
+        ib = jlo + ii;
+        jb = jlo + jj;
+    }
     const bool diag = ib == jb;
     constexpr int PER = TB * TB / 2 / NTHR_T;
     dbl2 va[PER], vb[PER];
@@ -1078,6 +1091,38 @@ __global__ void __launch_bounds__(256) k_split3(const double *A, int ld, int k0,
 // products), and three workgroups share a CU (3 x 48 KB of LDS) to keep
 // three stages in flight.  The six products of a 16x16 block, their order
 // and the split are those of k_trail<2>: the same factor bit for bit.
+// 128x128 block (IB, JB) of workgroup b in the 2x2-tile blocks from jlo of
+// a trailing-update target (see k_trail): the triangle in 8x8 groups of
+// blocks, workgroups going round-robin over the eight XCDs (b % 8) and XCD x
+// taking groups x, x + 8, ..., 64 consecutive workgroups of its own per
+// group, so a group's 16 row and 16 column panel tiles are read into one
+// L2; the strip row by row.
+__device__ __forceinline__ bool blk128(int b, int jlo, int jhi, int nt, int &IB, int &JB) {
+    const int S2 = (nt - jlo + 1) / 2;
+    if (jhi >= nt) {
+        const int SG = (S2 + 7) / 8, l = b >> 3, G = (l >> 6) * 8 + (b & 7);
+        if (G >= SG * (SG + 1) / 2) return false;
+        int GI, GJ;
+        tri_index(G, GI, GJ);
+        IB = 8 * GI + ((l >> 3) & 7);
+        JB = 8 * GJ + (l & 7);
+        return IB < S2 && JB <= IB;
+    }
+    const int SJ = (jhi - jlo + 1) / 2;
+    if (b >= SJ * (SJ + 1) / 2 + (S2 - SJ) * SJ) return false;
+    trap_index(b, SJ, IB, JB);
+    return true;
+}
+int blk128_grid(int jlo, int jhi, int nt) {
+    const int S2 = (nt - jlo + 1) / 2;
+    if (jhi >= nt) {
+        const int SG = (S2 + 7) / 8;
+        return (SG * (SG + 1) / 2 + 7) / 8 * 512;
+    }
+    const int SJ = (jhi - jlo + 1) / 2;
+    return SJ * (SJ + 1) / 2 + (S2 - SJ) * SJ;
+}
+
 typedef __attribute__((address_space(3))) void lds_void;
 __device__ __forceinline__ int bf_slot(int op, int tile, int s, int rb) { return ((op * 2 + tile) * 3 + s) * 4 + rb; }
 __device__ __forceinline__ const __bf16 *bf_src(const __bf16 *sp, int row, int nb, int t, int s, int rb, int ks,
@@ -1169,21 +1214,17 @@ __device__ __forceinline__ void bf_body(const __bf16 *sp, int ra, int rbw, int n
         }
     }
 }
-// Workgroups go round-robin over the eight XCDs (b % 8); XCD x takes groups
-// x, x + 8, ... of 8x8 blocks, 64 consecutive workgroups of its own per
-// group, so a group's 16 row and 16 column panel tiles are read into one L2.
-__global__ void __launch_bounds__(256, 3) k_trail_bf(double *A, int ld, const __bf16 *sp, int nb, int j0, int nt) {
+// j0: first row of the split panel tiles; the target tiles as in k_trail.
+__global__ void __launch_bounds__(256, 3) k_trail_bf(double *A, int ld, const __bf16 *sp, int nb, int j0, int jlo,
+                                                    int jhi, int nt) {
     __shared__ __attribute__((aligned(16))) __bf16 S[48 * 512];
-    const int S2 = (nt - j0 + 1) / 2, SG = (S2 + 7) / 8, l = blockIdx.x >> 3, G = (l >> 6) * 8 + (blockIdx.x & 7);
-    if (G >= SG * (SG + 1) / 2) return;
-    int GI, GJ;
-    tri_index(G, GI, GJ);
-    const int IB = 8 * GI + ((l >> 3) & 7), JB = 8 * GJ + (l & 7);
-    if (IB >= S2 || JB > IB) return;
+    int IB, JB;
+    if (!blk128(blockIdx.x, jlo, jhi, nt, IB, JB)) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
-    const int nrow = nt - j0, ra = 2 * IB, rbw = 2 * JB;  // tile rows relative to j0
+    // tile rows relative to j0 (the split rows) and the target tiles
+    const int nrow = nt - j0, ra = jlo - j0 + 2 * IB, rbw = jlo - j0 + 2 * JB;
     const int ib = j0 + ra + wr, jb = j0 + rbw + wc;
-    const bool active = ib < nt && jb <= ib;
+    const bool active = ib < nt && jb <= ib && jb < jhi;
     flt4 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -1328,18 +1369,16 @@ __device__ __forceinline__ void pk_body(const double *pk, int ra, int rbw, int n
     }
 }
 template <int MODE>
-__global__ void __launch_bounds__(256, 2) k_trail_pk(double *A, int ld, const double *pk, int nb, int j0, int nt) {
+__global__ void __launch_bounds__(256, 2) k_trail_pk(double *A, int ld, const double *pk, int nb, int j0, int jlo,
+                                                    int jhi, int nt) {
     __shared__ __attribute__((aligned(16))) double S[2 * 32 * 128];
-    const int S2 = (nt - j0 + 1) / 2, SG = (S2 + 7) / 8, l = blockIdx.x >> 3, G = (l >> 6) * 8 + (blockIdx.x & 7);
-    if (G >= SG * (SG + 1) / 2) return;
-    int GI, GJ;
-    tri_index(G, GI, GJ);
-    const int IB = 8 * GI + ((l >> 3) & 7), JB = 8 * GJ + (l & 7);
-    if (IB >= S2 || JB > IB) return;
+    int IB, JB;
+    if (!blk128(blockIdx.x, jlo, jhi, nt, IB, JB)) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
-    const int nrow = nt - j0, ra = 2 * IB, rbw = 2 * JB;  // tile rows relative to j0
+    // tile rows relative to j0 (the packed rows) and the target tiles
+    const int nrow = nt - j0, ra = jlo - j0 + 2 * IB, rbw = jlo - j0 + 2 * JB;
     const int ib = j0 + ra + wr, jb = j0 + rbw + wc;
-    const bool active = ib < nt && jb <= ib;
+    const bool active = ib < nt && jb <= ib && jb < jhi;
     typedef typename AccOf<MODE>::T V;
     V acc[4][4];
 #pragma unroll
@@ -1581,30 +1620,44 @@ void chol_column(const CholWork &c, int kb, int jcols, hipStream_t q) {
         k_step<MODE><<<grid, 256, 0, q>>>(c.A, c.ld, kb, jcols, c.Linv, c.neg);
     }
 }
+// The panel tiles of a super-panel packed (FP64 / FP32) or split (BF16) once
+// for the trailing update, where that form pays: in FP64 from 192 trailing
+// block rows, in FP32 from about 96 (fewer 128x128 blocks than workgroup
+// slots below that), in BF16 always (the direct form splits every operand
+// fragment in every workgroup).  Returns whether it did.
 template <int MODE>
-void chol_trail(const CholWork &c, int k0, int nb, hipStream_t q) {
+bool chol_pack(const CholWork &c, int k0, int nb, hipStream_t q) {
     const int nt = c.mpad / TB, j0 = k0 + nb;
-    // the 128x128 blocks pay in FP64 from 192 trailing block rows, in FP32
-    // from about 96 (fewer blocks than workgroup slots below that); in BF16
-    // always (the direct form splits every operand fragment in every
-    // workgroup)
     const int min_rows = c.pack_min_rows >= 0 ? c.pack_min_rows : MODE == 0 ? 192 : MODE == 1 ? 96 : 0;
-    if (c.sp && nb <= c.sp_nb && nt - j0 >= min_rows) {
-        // panel tiles packed (split) once, 128x128 blocks in 8x8 groups
-        const int S2 = (nt - j0 + 1) / 2, SG = (S2 + 7) / 8, grid = (SG * (SG + 1) / 2 + 7) / 8 * 512;
-        if constexpr (MODE == 2) {
-            __bf16 *sp = static_cast<__bf16 *>(c.sp);
-            k_split3<<<(nt - j0) * nb, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, sp);
-            k_trail_bf<<<grid, 256, 0, q>>>(c.A, c.ld, sp, nb, j0, nt);
-        } else {
-            double *pk = static_cast<double *>(c.sp);
-            k_pack64<<<(nt - j0) * nb, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, pk);
-            k_trail_pk<MODE><<<grid, 256, 0, q>>>(c.A, c.ld, pk, nb, j0, nt);
-        }
+    if (!c.sp || nb > c.sp_nb || nt - j0 < min_rows) return false;
+    if constexpr (MODE == 2)
+        k_split3<<<(nt - j0) * nb, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, static_cast<__bf16 *>(c.sp));
+    else
+        k_pack64<<<(nt - j0) * nb, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, static_cast<double *>(c.sp));
+    return true;
+}
+// The super-panel's update of the trailing tiles in columns jlo .. jhi - 1
+// (jhi == nt: the triangle from jlo; see k_trail).
+template <int MODE>
+void chol_trail(const CholWork &c, int k0, int nb, int jlo, int jhi, bool packed, hipStream_t q) {
+    const int nt = c.mpad / TB, j0 = k0 + nb;
+    if (packed) {
+        const int grid = blk128_grid(jlo, jhi, nt);
+        if constexpr (MODE == 2)
+            k_trail_bf<<<grid, 256, 0, q>>>(c.A, c.ld, static_cast<const __bf16 *>(c.sp), nb, j0, jlo, jhi, nt);
+        else
+            k_trail_pk<MODE><<<grid, 256, 0, q>>>(c.A, c.ld, static_cast<const double *>(c.sp), nb, j0, jlo, jhi, nt);
         return;
     }
-    const int SM = (nt - j0 + 7) / 8;
-    k_trail<MODE><<<SM * (SM + 1) / 2 * 64, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, nt);
+    int grid;
+    if (jhi >= nt) {
+        const int SM = (nt - jlo + 7) / 8;
+        grid = SM * (SM + 1) / 2 * 64;
+    } else {
+        const int jc = jhi - jlo;
+        grid = jc * (jc + 1) / 2 + (nt - jlo - jc) * jc;
+    }
+    k_trail<MODE><<<grid, 256, 0, q>>>(c.A, c.ld, k0, nb, jlo, jhi, nt);
 }
 }  // namespace
 
@@ -1632,24 +1685,50 @@ void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_a
         if (gate && kb == gate_kb) (void)hipEventRecord(gate, q);
     };
     const int NB = chol_panel_tiles(c);
+    // look-ahead: a super-panel's trailing update is split into the next
+    // super-panel's columns (on q, ahead of the next panel) and the rest of
+    // the trailing matrix (on c.la_q beside the next panel's columns, which
+    // touch none of its tiles); the next packing waits for the rest, which
+    // reads the packed tiles.  Same tiles, same sums: the same factor.
+    const bool la = NB > 0 && c.la_q && c.la_pack && c.la_rest && !mark;
+    bool rest_pending = false;
     int k0 = 0;
     if (NB > 0) {
         // super-panels while a whole one and a trailing matrix remain; the
         // panel's own columns are factored in FP64 in every mode (the
-        // reduced-precision modes apply to the trailing update k_trail)
+        // reduced-precision modes apply to the trailing update)
         for (; k0 + NB < nt; k0 += NB) {
             const int kend = k0 + NB;
             for (int kb = k0; kb < kend; kb++) column(kb, kend - 1 - kb, 0);
+            if (rest_pending) (void)hipStreamWaitEvent(q, c.la_rest, 0);
+            rest_pending = false;
             M_(K_CHOL_TRAIL, 0);
-            if (c.mode == 0) chol_trail<0>(c, k0, NB, q);
-            else if (c.mode == 1) chol_trail<1>(c, k0, NB, q);
-            else chol_trail<2>(c, k0, NB, q);
+            bool packed;
+            if (c.mode == 0) packed = chol_pack<0>(c, k0, NB, q);
+            else if (c.mode == 1) packed = chol_pack<1>(c, k0, NB, q);
+            else packed = chol_pack<2>(c, k0, NB, q);
+            auto trail = [&](int jlo, int jhi, hipStream_t s) {
+                if (c.mode == 0) chol_trail<0>(c, k0, NB, jlo, jhi, packed, s);
+                else if (c.mode == 1) chol_trail<1>(c, k0, NB, jlo, jhi, packed, s);
+                else chol_trail<2>(c, k0, NB, jlo, jhi, packed, s);
+            };
+            if (la && kend + NB < nt) {
+                (void)hipEventRecord(c.la_pack, q);
+                (void)hipStreamWaitEvent(c.la_q, c.la_pack, 0);
+                trail(kend + NB, nt, c.la_q);
+                (void)hipEventRecord(c.la_rest, c.la_q);
+                rest_pending = true;
+                trail(kend, kend + NB, q);
+            } else {
+                trail(kend, nt, q);
+            }
             M_(K_CHOL_TRAIL, 1);
             M_(K_CHOL_DIAG, 0);
             k_diag<<<1, 256, 0, q>>>(c.A, c.ld, kend, c.Linv, c.neg, 0);
             M_(K_CHOL_DIAG, 1);
         }
     }
+    if (rest_pending) (void)hipStreamWaitEvent(q, c.la_rest, 0);
     // one launch (or two) per block column, each updating the whole trailing matrix
     for (int kb = k0; kb + 1 < nt; kb++) column(kb, nt - kb - 1, c.mode);
     if (gate && gate_kb >= nt - 1) (void)hipEventRecord(gate, q);

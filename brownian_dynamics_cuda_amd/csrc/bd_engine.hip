@@ -371,6 +371,14 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
             return bail(fail(BD_E_HIP, "hipStreamCreate failed"));
         c->fstr[0].b[i] = c->fstr[1].b[i] = b;
     }
+    // the factor's look-ahead runs on a force-branch stream, idle while the
+    // factor runs (force pass 1 stays on the main stream)
+    if (!getenv("BD_LOOKAHEAD") || atoi(getenv("BD_LOOKAHEAD")) != 0) {
+        if (hipEventCreateWithFlags(&c->chol.la_pack, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->chol.la_rest, hipEventDisableTiming) != hipSuccess)
+            return bail(fail(BD_E_HIP, "hipEventCreate failed"));
+        c->chol.la_q = c->fstr[0].b[0];
+    }
     for (auto &fs : c->fstr) {
         bool ok = hipEventCreateWithFlags(&fs.fork, hipEventDisableTiming) == hipSuccess &&
                   hipEventCreateWithFlags(&fs.mech, hipEventDisableTiming) == hipSuccess;
@@ -606,6 +614,8 @@ int bd_destroy(bd_context *c) {
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_gate) (void)hipEventDestroy(c->ev_gate);
+    if (c->chol.la_pack) (void)hipEventDestroy(c->chol.la_pack);
+    if (c->chol.la_rest) (void)hipEventDestroy(c->chol.la_rest);
     for (int i = 0; i < 3; i++)
         if (c->fstr[0].b[i]) {
             (void)hipStreamSynchronize(c->fstr[0].b[i]);

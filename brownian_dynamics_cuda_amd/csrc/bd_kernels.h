@@ -93,6 +93,10 @@ struct CholWork {
     // smallest trailing matrix (block rows) that takes the packed form; -1:
     // the mode's default
     int pack_min_rows;
+    // look-ahead across super-panels (all three set: on): the stream that
+    // takes the rest of each trailing update, and the events that order it
+    hipStream_t la_q;
+    hipEvent_t la_pack, la_rest;
 };
 // bytes of the packed / split panel buffer for leading dimension ld and
 // panels of nb
