@@ -45,7 +45,8 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
                    double *Epart, int terms, hipStream_t q, MarkFn mark = nullptr, void *arg = nullptr,
                    const ForceStreams *fs = nullptr);
 int energy_slots(const Stat &s);
-void launch_energy_reduce(const double *Epart, int ne, double *out, hipStream_t q);
+void launch_energy_reduce(const double *Epart, int ne, double *out, hipStream_t q,
+                          double *const *extra = nullptr);
 void launch_fill(double *p, long long n, double v, hipStream_t q);
 void launch_fill_int(int *p, long long n, int v, hipStream_t q);
 

@@ -191,6 +191,11 @@ def test_device_noise_path_equals_host_noise_path():
         eng.synchronize()
         e2 = float(d_e.download(1)[0])
         s2 = eng.get_state()
+        # a later host-buffer step leaves the earlier call's energy slot alone
+        d_e.upload(np.array([-7.0]))
+        eng.step(rr, p)
+        eng.synchronize()
+        assert float(d_e.download(1)[0]) == -7.0
     assert e1 == e2 and np.array_equal(s1.r, s2.r) and np.array_equal(s1.r_lh, s2.r_lh)
 
 
