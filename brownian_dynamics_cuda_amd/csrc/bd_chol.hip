@@ -98,15 +98,27 @@ __device__ __forceinline__ void diag_factor_inv(double *T, double *LP, double *X
 // FACTOR: workgroup 0 (tile (0,0)) then factors and inverts its tile in
 // place of a separate first-tile launch, beside the other tiles' assembly
 // (every workgroup then carries the factor's LDS).
+// xs (optional) takes the positions of the hydrodynamic beads D is built
+// from, written by the diagonal tiles' workgroups (bead b by the tile of row
+// 3b), for the refinement's matrix-free D u.
 template <bool FACTOR>
 __global__ void __launch_bounds__(256) k_rpy(Stat s, Geo g, const double *rad_all, double *A, int m, int ld,
-                                             double a1, double a2, int *neg, double *Linv) {
+                                             double a1, double a2, int *neg, double *Linv, double *xs) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *neg = 0;  // negative-pivot count of the factor that follows
     __shared__ double T[TB * LDP];
     __shared__ double LP[FACTOR ? TB * TB : 1], X[FACTOR ? TB * LDP : 1];
     int ti, tj;
     tri_index(blockIdx.x, ti, tj);
     const int r0 = ti * TB, c0 = tj * TB, nhi = m / 3;
+    if (xs && ti == tj) {
+        const int b = (r0 + 2) / 3 + (int)threadIdx.x;  // first bead whose row 3b is in the tile
+        if (threadIdx.x < TB / 3 + 1 && 3 * b < r0 + TB && b < nhi) {
+            const double3 r = hi_pos(s, g, b);
+            xs[3 * b] = r.x;
+            xs[3 * b + 1] = r.y;
+            xs[3 * b + 2] = r.z;
+        }
+    }
     // beads whose rows / columns meet the tile
     const int bi0 = r0 / 3, bj0 = c0 / 3;
     const int nbi = max(0, min((r0 + TB - 1) / 3, nhi - 1) - bi0 + 1);
@@ -1474,14 +1486,6 @@ __global__ void __launch_bounds__(256) k_noise(const double *A, int ld, int m, i
 // product D u from the bead positions D was assembled from (k_dmul, the RPY
 // blocks of k_rpy), and the mean taken in k_noise beside L~ xi.
 // ---------------------------------------------------------------------------
-__global__ void k_hi_snapshot(Stat s, Geo g, int nhi, double *xs) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nhi) return;
-    const double3 r = hi_pos(s, g, b);
-    xs[3 * b] = r.x;
-    xs[3 * b + 1] = r.y;
-    xs[3 * b + 2] = r.z;
-}
 
 __global__ void k_refine_init(const double *p, int m, int mpad, double *pw) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1592,15 +1596,16 @@ __global__ void __launch_bounds__(256) k_dmul(const double *xs, const double *ra
 void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, const CholWork &c, hipStream_t q,
                 bool factor_first) {
     const int nt = c.mpad / TB;
-    if (c.xs && c.m > 0) k_hi_snapshot<<<nblk(c.m / 3, 256), 256, 0, q>>>(s, g, c.m / 3, c.xs);
     // the first tile's factor inside the assembly launch saves a launch on
     // the factor's chain, but gives every workgroup the factor's LDS (one
     // per CU): from 64 block columns the assembly takes its own launch
     // (four workgroups per CU) and k_diag the first tile
     if (factor_first && nt < 64) {
-        k_rpy<true><<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2, c.neg, c.Linv);
+        k_rpy<true><<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2, c.neg, c.Linv,
+                                                     c.xs);
     } else {
-        k_rpy<false><<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2, c.neg, c.Linv);
+        k_rpy<false><<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2, c.neg, c.Linv,
+                                                      c.xs);
         if (factor_first) launch_diag(c, 0, false, q);
     }
     (void)n_hi;
