@@ -98,27 +98,15 @@ __device__ __forceinline__ void diag_factor_inv(double *T, double *LP, double *X
 // FACTOR: workgroup 0 (tile (0,0)) then factors and inverts its tile in
 // place of a separate first-tile launch, beside the other tiles' assembly
 // (every workgroup then carries the factor's LDS).
-// xs (optional) takes the positions of the hydrodynamic beads D is built
-// from, written by the diagonal tiles' workgroups (bead b by the tile of row
-// 3b), for the refinement's matrix-free D u.
 template <bool FACTOR>
 __global__ void __launch_bounds__(256) k_rpy(Stat s, Geo g, const double *rad_all, double *A, int m, int ld,
-                                             double a1, double a2, int *neg, double *Linv, double *xs) {
+                                             double a1, double a2, int *neg, double *Linv) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *neg = 0;  // negative-pivot count of the factor that follows
     __shared__ double T[TB * LDP];
     __shared__ double LP[FACTOR ? TB * TB : 1], X[FACTOR ? TB * LDP : 1];
     int ti, tj;
     tri_index(blockIdx.x, ti, tj);
     const int r0 = ti * TB, c0 = tj * TB, nhi = m / 3;
-    if (xs && ti == tj) {
-        const int b = (r0 + 2) / 3 + (int)threadIdx.x;  // first bead whose row 3b is in the tile
-        if (threadIdx.x < TB / 3 + 1 && 3 * b < r0 + TB && b < nhi) {
-            const double3 r = hi_pos(s, g, b);
-            xs[3 * b] = r.x;
-            xs[3 * b + 1] = r.y;
-            xs[3 * b + 2] = r.z;
-        }
-    }
     // beads whose rows / columns meet the tile
     const int bi0 = r0 / 3, bj0 = c0 / 3;
     const int nbi = max(0, min((r0 + TB - 1) / 3, nhi - 1) - bi0 + 1);
@@ -1002,26 +990,13 @@ __device__ __forceinline__ void regs_to_lds(const dbl2 (&v)[TB * TB / 2 / NTHR_T
         S[r * LDP + c + 1] = v[i].y;
     }
 }
-// The target of a trailing-update launch is the lower tiles (i, j) with
-// jlo <= j < jhi, i from j to nt - 1: jhi == nt the whole triangle from jlo,
-// jhi < nt the strip of the next super-panel's columns, which is updated
-// ahead of the rest.
 template <int MODE>
-__global__ void __launch_bounds__(256, 2) k_trail(double *A, int ld, int k0, int nb, int jlo, int jhi, int nt) {
+__global__ void __launch_bounds__(256, 2) k_trail(double *A, int ld, int k0, int nb, int j0, int nt) {
     __shared__ double As[TB * LDP], Bs[TB * LDP];
-    int ib, jb;
-    if (jhi >= nt) {
-        int SI, SJ;
-        tri_index(blockIdx.x >> 6, SI, SJ);
-        ib = jlo + 8 * SI + ((blockIdx.x >> 3) & 7);
-        jb = jlo + 8 * SJ + (blockIdx.x & 7);
-        if (ib >= nt || jb > ib) return;
-    } else {
-        int ii, jj;
-        trap_index(blockIdx.x, jhi - jlo, ii, jj);
-        ib = jlo + ii;
-        jb = jlo + jj;
-    }
+    int SI, SJ;
+    tri_index(blockIdx.x >> 6, SI, SJ);
+    const int ib = j0 + 8 * SI + ((blockIdx.x >> 3) & 7), jb = j0 + 8 * SJ + (blockIdx.x & 7);
+    if (ib >= nt || jb > ib) return;
     const bool diag = ib == jb;
     constexpr int PER = TB * TB / 2 / NTHR_T;
     dbl2 va[PER], vb[PER];
@@ -1103,38 +1078,6 @@ __global__ void __launch_bounds__(256) k_split3(const double *A, int ld, int k0,
 // products), and three workgroups share a CU (3 x 48 KB of LDS) to keep
 // three stages in flight.  The six products of a 16x16 block, their order
 // and the split are those of k_trail<2>: the same factor bit for bit.
-// 128x128 block (IB, JB) of workgroup b in the 2x2-tile blocks from jlo of
-// a trailing-update target (see k_trail): the triangle in 8x8 groups of
-// blocks, workgroups going round-robin over the eight XCDs (b % 8) and XCD x
-// taking groups x, x + 8, ..., 64 consecutive workgroups of its own per
-// group, so a group's 16 row and 16 column panel tiles are read into one
-// L2; the strip row by row.
-__device__ __forceinline__ bool blk128(int b, int jlo, int jhi, int nt, int &IB, int &JB) {
-    const int S2 = (nt - jlo + 1) / 2;
-    if (jhi >= nt) {
-        const int SG = (S2 + 7) / 8, l = b >> 3, G = (l >> 6) * 8 + (b & 7);
-        if (G >= SG * (SG + 1) / 2) return false;
-        int GI, GJ;
-        tri_index(G, GI, GJ);
-        IB = 8 * GI + ((l >> 3) & 7);
-        JB = 8 * GJ + (l & 7);
-        return IB < S2 && JB <= IB;
-    }
-    const int SJ = (jhi - jlo + 1) / 2;
-    if (b >= SJ * (SJ + 1) / 2 + (S2 - SJ) * SJ) return false;
-    trap_index(b, SJ, IB, JB);
-    return true;
-}
-int blk128_grid(int jlo, int jhi, int nt) {
-    const int S2 = (nt - jlo + 1) / 2;
-    if (jhi >= nt) {
-        const int SG = (S2 + 7) / 8;
-        return (SG * (SG + 1) / 2 + 7) / 8 * 512;
-    }
-    const int SJ = (jhi - jlo + 1) / 2;
-    return SJ * (SJ + 1) / 2 + (S2 - SJ) * SJ;
-}
-
 typedef __attribute__((address_space(3))) void lds_void;
 __device__ __forceinline__ int bf_slot(int op, int tile, int s, int rb) { return ((op * 2 + tile) * 3 + s) * 4 + rb; }
 __device__ __forceinline__ const __bf16 *bf_src(const __bf16 *sp, int row, int nb, int t, int s, int rb, int ks,
@@ -1226,17 +1169,21 @@ __device__ __forceinline__ void bf_body(const __bf16 *sp, int ra, int rbw, int n
         }
     }
 }
-// j0: first row of the split panel tiles; the target tiles as in k_trail.
-__global__ void __launch_bounds__(256, 3) k_trail_bf(double *A, int ld, const __bf16 *sp, int nb, int j0, int jlo,
-                                                    int jhi, int nt) {
+// Workgroups go round-robin over the eight XCDs (b % 8); XCD x takes groups
+// x, x + 8, ... of 8x8 blocks, 64 consecutive workgroups of its own per
+// group, so a group's 16 row and 16 column panel tiles are read into one L2.
+__global__ void __launch_bounds__(256, 3) k_trail_bf(double *A, int ld, const __bf16 *sp, int nb, int j0, int nt) {
     __shared__ __attribute__((aligned(16))) __bf16 S[48 * 512];
-    int IB, JB;
-    if (!blk128(blockIdx.x, jlo, jhi, nt, IB, JB)) return;
+    const int S2 = (nt - j0 + 1) / 2, SG = (S2 + 7) / 8, l = blockIdx.x >> 3, G = (l >> 6) * 8 + (blockIdx.x & 7);
+    if (G >= SG * (SG + 1) / 2) return;
+    int GI, GJ;
+    tri_index(G, GI, GJ);
+    const int IB = 8 * GI + ((l >> 3) & 7), JB = 8 * GJ + (l & 7);
+    if (IB >= S2 || JB > IB) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
-    // tile rows relative to j0 (the split rows) and the target tiles
-    const int nrow = nt - j0, ra = jlo - j0 + 2 * IB, rbw = jlo - j0 + 2 * JB;
+    const int nrow = nt - j0, ra = 2 * IB, rbw = 2 * JB;  // tile rows relative to j0
     const int ib = j0 + ra + wr, jb = j0 + rbw + wc;
-    const bool active = ib < nt && jb <= ib && jb < jhi;
+    const bool active = ib < nt && jb <= ib;
     flt4 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -1381,16 +1328,18 @@ __device__ __forceinline__ void pk_body(const double *pk, int ra, int rbw, int n
     }
 }
 template <int MODE>
-__global__ void __launch_bounds__(256, 2) k_trail_pk(double *A, int ld, const double *pk, int nb, int j0, int jlo,
-                                                    int jhi, int nt) {
+__global__ void __launch_bounds__(256, 2) k_trail_pk(double *A, int ld, const double *pk, int nb, int j0, int nt) {
     __shared__ __attribute__((aligned(16))) double S[2 * 32 * 128];
-    int IB, JB;
-    if (!blk128(blockIdx.x, jlo, jhi, nt, IB, JB)) return;
+    const int S2 = (nt - j0 + 1) / 2, SG = (S2 + 7) / 8, l = blockIdx.x >> 3, G = (l >> 6) * 8 + (blockIdx.x & 7);
+    if (G >= SG * (SG + 1) / 2) return;
+    int GI, GJ;
+    tri_index(G, GI, GJ);
+    const int IB = 8 * GI + ((l >> 3) & 7), JB = 8 * GJ + (l & 7);
+    if (IB >= S2 || JB > IB) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
-    // tile rows relative to j0 (the packed rows) and the target tiles
-    const int nrow = nt - j0, ra = jlo - j0 + 2 * IB, rbw = jlo - j0 + 2 * JB;
+    const int nrow = nt - j0, ra = 2 * IB, rbw = 2 * JB;  // tile rows relative to j0
     const int ib = j0 + ra + wr, jb = j0 + rbw + wc;
-    const bool active = ib < nt && jb <= ib && jb < jhi;
+    const bool active = ib < nt && jb <= ib;
     typedef typename AccOf<MODE>::T V;
     V acc[4][4];
 #pragma unroll
@@ -1486,6 +1435,14 @@ __global__ void __launch_bounds__(256) k_noise(const double *A, int ld, int m, i
 // product D u from the bead positions D was assembled from (k_dmul, the RPY
 // blocks of k_rpy), and the mean taken in k_noise beside L~ xi.
 // ---------------------------------------------------------------------------
+__global__ void k_hi_snapshot(Stat s, Geo g, int nhi, double *xs) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nhi) return;
+    const double3 r = hi_pos(s, g, b);
+    xs[3 * b] = r.x;
+    xs[3 * b + 1] = r.y;
+    xs[3 * b + 2] = r.z;
+}
 
 __global__ void k_refine_init(const double *p, int m, int mpad, double *pw) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1596,16 +1553,15 @@ __global__ void __launch_bounds__(256) k_dmul(const double *xs, const double *ra
 void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, const CholWork &c, hipStream_t q,
                 bool factor_first) {
     const int nt = c.mpad / TB;
+    if (c.xs && c.m > 0) k_hi_snapshot<<<nblk(c.m / 3, 256), 256, 0, q>>>(s, g, c.m / 3, c.xs);
     // the first tile's factor inside the assembly launch saves a launch on
     // the factor's chain, but gives every workgroup the factor's LDS (one
     // per CU): from 64 block columns the assembly takes its own launch
     // (four workgroups per CU) and k_diag the first tile
     if (factor_first && nt < 64) {
-        k_rpy<true><<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2, c.neg, c.Linv,
-                                                     c.xs);
+        k_rpy<true><<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2, c.neg, c.Linv);
     } else {
-        k_rpy<false><<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2, c.neg, c.Linv,
-                                                      c.xs);
+        k_rpy<false><<<nt * (nt + 1) / 2, 256, 0, q>>>(s, g, rad_all, c.A, c.m, c.ld, s.a1, s.a2, c.neg, c.Linv);
         if (factor_first) launch_diag(c, 0, false, q);
     }
     (void)n_hi;
@@ -1625,44 +1581,30 @@ void chol_column(const CholWork &c, int kb, int jcols, hipStream_t q) {
         k_step<MODE><<<grid, 256, 0, q>>>(c.A, c.ld, kb, jcols, c.Linv, c.neg);
     }
 }
-// The panel tiles of a super-panel packed (FP64 / FP32) or split (BF16) once
-// for the trailing update, where that form pays: in FP64 from 192 trailing
-// block rows, in FP32 from about 96 (fewer 128x128 blocks than workgroup
-// slots below that), in BF16 always (the direct form splits every operand
-// fragment in every workgroup).  Returns whether it did.
 template <int MODE>
-bool chol_pack(const CholWork &c, int k0, int nb, hipStream_t q) {
+void chol_trail(const CholWork &c, int k0, int nb, hipStream_t q) {
     const int nt = c.mpad / TB, j0 = k0 + nb;
+    // the 128x128 blocks pay in FP64 from 192 trailing block rows, in FP32
+    // from about 96 (fewer blocks than workgroup slots below that); in BF16
+    // always (the direct form splits every operand fragment in every
+    // workgroup)
     const int min_rows = c.pack_min_rows >= 0 ? c.pack_min_rows : MODE == 0 ? 192 : MODE == 1 ? 96 : 0;
-    if (!c.sp || nb > c.sp_nb || nt - j0 < min_rows) return false;
-    if constexpr (MODE == 2)
-        k_split3<<<(nt - j0) * nb, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, static_cast<__bf16 *>(c.sp));
-    else
-        k_pack64<<<(nt - j0) * nb, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, static_cast<double *>(c.sp));
-    return true;
-}
-// The super-panel's update of the trailing tiles in columns jlo .. jhi - 1
-// (jhi == nt: the triangle from jlo; see k_trail).
-template <int MODE>
-void chol_trail(const CholWork &c, int k0, int nb, int jlo, int jhi, bool packed, hipStream_t q) {
-    const int nt = c.mpad / TB, j0 = k0 + nb;
-    if (packed) {
-        const int grid = blk128_grid(jlo, jhi, nt);
-        if constexpr (MODE == 2)
-            k_trail_bf<<<grid, 256, 0, q>>>(c.A, c.ld, static_cast<const __bf16 *>(c.sp), nb, j0, jlo, jhi, nt);
-        else
-            k_trail_pk<MODE><<<grid, 256, 0, q>>>(c.A, c.ld, static_cast<const double *>(c.sp), nb, j0, jlo, jhi, nt);
+    if (c.sp && nb <= c.sp_nb && nt - j0 >= min_rows) {
+        // panel tiles packed (split) once, 128x128 blocks in 8x8 groups
+        const int S2 = (nt - j0 + 1) / 2, SG = (S2 + 7) / 8, grid = (SG * (SG + 1) / 2 + 7) / 8 * 512;
+        if constexpr (MODE == 2) {
+            __bf16 *sp = static_cast<__bf16 *>(c.sp);
+            k_split3<<<(nt - j0) * nb, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, sp);
+            k_trail_bf<<<grid, 256, 0, q>>>(c.A, c.ld, sp, nb, j0, nt);
+        } else {
+            double *pk = static_cast<double *>(c.sp);
+            k_pack64<<<(nt - j0) * nb, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, pk);
+            k_trail_pk<MODE><<<grid, 256, 0, q>>>(c.A, c.ld, pk, nb, j0, nt);
+        }
         return;
     }
-    int grid;
-    if (jhi >= nt) {
-        const int SM = (nt - jlo + 7) / 8;
-        grid = SM * (SM + 1) / 2 * 64;
-    } else {
-        const int jc = jhi - jlo;
-        grid = jc * (jc + 1) / 2 + (nt - jlo - jc) * jc;
-    }
-    k_trail<MODE><<<grid, 256, 0, q>>>(c.A, c.ld, k0, nb, jlo, jhi, nt);
+    const int SM = (nt - j0 + 7) / 8;
+    k_trail<MODE><<<SM * (SM + 1) / 2 * 64, 256, 0, q>>>(c.A, c.ld, k0, nb, j0, nt);
 }
 }  // namespace
 
@@ -1690,50 +1632,24 @@ void launch_cholesky(const CholWork &c, hipStream_t q, MarkFn mark, void *mark_a
         if (gate && kb == gate_kb) (void)hipEventRecord(gate, q);
     };
     const int NB = chol_panel_tiles(c);
-    // look-ahead: a super-panel's trailing update is split into the next
-    // super-panel's columns (on q, ahead of the next panel) and the rest of
-    // the trailing matrix (on c.la_q beside the next panel's columns, which
-    // touch none of its tiles); the next packing waits for the rest, which
-    // reads the packed tiles.  Same tiles, same sums: the same factor.
-    const bool la = NB > 0 && c.la_q && c.la_pack && c.la_rest && !mark;
-    bool rest_pending = false;
     int k0 = 0;
     if (NB > 0) {
         // super-panels while a whole one and a trailing matrix remain; the
         // panel's own columns are factored in FP64 in every mode (the
-        // reduced-precision modes apply to the trailing update)
+        // reduced-precision modes apply to the trailing update k_trail)
         for (; k0 + NB < nt; k0 += NB) {
             const int kend = k0 + NB;
             for (int kb = k0; kb < kend; kb++) column(kb, kend - 1 - kb, 0);
-            if (rest_pending) (void)hipStreamWaitEvent(q, c.la_rest, 0);
-            rest_pending = false;
             M_(K_CHOL_TRAIL, 0);
-            bool packed;
-            if (c.mode == 0) packed = chol_pack<0>(c, k0, NB, q);
-            else if (c.mode == 1) packed = chol_pack<1>(c, k0, NB, q);
-            else packed = chol_pack<2>(c, k0, NB, q);
-            auto trail = [&](int jlo, int jhi, hipStream_t s) {
-                if (c.mode == 0) chol_trail<0>(c, k0, NB, jlo, jhi, packed, s);
-                else if (c.mode == 1) chol_trail<1>(c, k0, NB, jlo, jhi, packed, s);
-                else chol_trail<2>(c, k0, NB, jlo, jhi, packed, s);
-            };
-            if (la && kend + NB < nt) {
-                (void)hipEventRecord(c.la_pack, q);
-                (void)hipStreamWaitEvent(c.la_q, c.la_pack, 0);
-                trail(kend + NB, nt, c.la_q);
-                (void)hipEventRecord(c.la_rest, c.la_q);
-                rest_pending = true;
-                trail(kend, kend + NB, q);
-            } else {
-                trail(kend, nt, q);
-            }
+            if (c.mode == 0) chol_trail<0>(c, k0, NB, q);
+            else if (c.mode == 1) chol_trail<1>(c, k0, NB, q);
+            else chol_trail<2>(c, k0, NB, q);
             M_(K_CHOL_TRAIL, 1);
             M_(K_CHOL_DIAG, 0);
             k_diag<<<1, 256, 0, q>>>(c.A, c.ld, kend, c.Linv, c.neg, 0);
             M_(K_CHOL_DIAG, 1);
         }
     }
-    if (rest_pending) (void)hipStreamWaitEvent(q, c.la_rest, 0);
     // one launch (or two) per block column, each updating the whole trailing matrix
     for (int kb = k0; kb + 1 < nt; kb++) column(kb, nt - kb - 1, c.mode);
     if (gate && gate_kb >= nt - 1) (void)hipEventRecord(gate, q);

@@ -88,9 +88,6 @@ struct bd_context {
     ForceScratch fw{};
     CholWork chol{};
     double *rd = nullptr, *p = nullptr, *rr = nullptr, *energy = nullptr, *Epart = nullptr;
-    // device word holding where the step also writes its energy (null: only
-    // to energy); set per call by k_stage, read by the captured reduction
-    double **eslot = nullptr;
     double *rad_all = nullptr;
     int ne = 0;
     bool state_set = false;
@@ -236,7 +233,7 @@ void enqueue_step(bd_context *c, bool refactor) {
     M(K_BETAS, 0); launch_betas(s, c->cur, 0, 1, q); M(K_BETAS, 1);
     // force pass 1 at x0 and its energy (3556-3615)
     enqueue_forces(c, c->cur, c->f0, c->Epart, BD_TERM_ALL, 0);
-    M(K_EREDUCE, 0); launch_energy_reduce(c->Epart, c->ne, c->energy, q, c->eslot); M(K_EREDUCE, 1);
+    M(K_EREDUCE, 0); launch_energy_reduce(c->Epart, c->ne, c->energy, q); M(K_EREDUCE, 1);
     // predictor (3617-3619)
     M(K_ROTATE, 0); launch_rotate(s, c->cur, c->pred, c->f0.T, c->rr, q); M(K_ROTATE, 1);
     if (qf != q) {
@@ -373,14 +370,6 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
         if (hipStreamCreateWithFlags(&b, hipStreamNonBlocking) != hipSuccess)
             return bail(fail(BD_E_HIP, "hipStreamCreate failed"));
         c->fstr[0].b[i] = c->fstr[1].b[i] = b;
-    }
-    // the factor's look-ahead runs on a force-branch stream, idle while the
-    // factor runs (force pass 1 stays on the main stream)
-    if (!getenv("BD_LOOKAHEAD") || atoi(getenv("BD_LOOKAHEAD")) != 0) {
-        if (hipEventCreateWithFlags(&c->chol.la_pack, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&c->chol.la_rest, hipEventDisableTiming) != hipSuccess)
-            return bail(fail(BD_E_HIP, "hipEventCreate failed"));
-        c->chol.la_q = c->fstr[0].b[0];
     }
     for (auto &fs : c->fstr) {
         bool ok = hipEventCreateWithFlags(&fs.fork, hipEventDisableTiming) == hipSuccess &&
@@ -576,8 +565,6 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     r |= c->alloc(&c->p, 3 * nD_cap);
     r |= c->alloc(&c->rr, 3 * n);
     r |= c->alloc(&c->energy, 1);
-    r |= c->alloc(&c->eslot, 1);
-    if (!r && hipMemset(c->eslot, 0, sizeof(double *)) != hipSuccess) r = BD_E_HIP;
     c->ne = energy_slots(s);
     {
         Stat cap = s;
@@ -619,8 +606,6 @@ int bd_destroy(bd_context *c) {
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_gate) (void)hipEventDestroy(c->ev_gate);
-    if (c->chol.la_pack) (void)hipEventDestroy(c->chol.la_pack);
-    if (c->chol.la_rest) (void)hipEventDestroy(c->chol.la_rest);
     for (int i = 0; i < 3; i++)
         if (c->fstr[0].b[i]) {
             (void)hipStreamSynchronize(c->fstr[0].b[i]);
@@ -763,22 +748,6 @@ int bd_get_derived(bd_context *c, bd_derived *d) {
     return e ? BD_E_HIP : BD_OK;
 }
 
-// A step's inputs in one launch: the noise from device buffers (either may
-// be null: already in place) and the address the step's energy also goes to.
-__global__ void k_stage(double *rr, double *p, const double *src_rr, const double *src_p, int n3, int nd3,
-                        double **eslot, double *e_dst) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (src_rr && i < n3) rr[i] = src_rr[i];
-    if (src_p && i < nd3) p[i] = src_p[i];
-    if (i == 0) *eslot = e_dst;
-}
-static int stage_inputs(bd_context *c, const double *d_rr, const double *d_p, double *d_energy) {
-    const int n3 = 3 * c->n, nd3 = c->n_D3, m = std::max(n3, nd3);
-    k_stage<<<(m + 255) / 256, 256, 0, c->stream>>>(c->rr, c->p, d_rr, d_p, n3, nd3, c->eslot, d_energy);
-    HIPCHK(hipGetLastError());
-    return BD_OK;
-}
-
 static int run_step(bd_context *c) {
     const bool refactor = !c->factor_fresh || c->rp_count % c->rp_freq == 0;
     c->rp_count++;
@@ -811,8 +780,7 @@ int bd_step(bd_context *c, const double *rr, const double *p, double *energy) {
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(c->rr, rr, sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->p, p, sizeof(double) * c->n_D3, hipMemcpyHostToDevice, c->stream));
-    int e = stage_inputs(c, nullptr, nullptr, nullptr);
-    if (!e) e = run_step(c);
+    int e = run_step(c);
     if (e) return e;
     if (energy) HIPCHK(hipMemcpyAsync(energy, c->energy, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     return sync_check(c);
@@ -823,11 +791,13 @@ int bd_step_device(bd_context *c, const double *d_rr, const double *d_p, double 
     if (!c->state_set) return fail(BD_E_STATE, "bd_set_state must be called before bd_step_device");
     if (!d_rr || !d_p) return fail(BD_E_ARG, "d_rr and d_p are required");
     HIPCHK(hipSetDevice(c->device));
-    // the noise copied in and the energy's destination set in one launch;
-    // the step's energy reduction writes d_energy itself
-    int e = stage_inputs(c, d_rr, d_p, d_energy);
-    if (!e) e = run_step(c);
-    return e;
+    HIPCHK(hipMemcpyAsync(c->rr, d_rr, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->p, d_p, sizeof(double) * c->n_D3, hipMemcpyDeviceToDevice, c->stream));
+    int e = run_step(c);
+    if (e) return e;
+    if (d_energy)
+        HIPCHK(hipMemcpyAsync(d_energy, c->energy, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return BD_OK;
 }
 
 int bd_run_steps(bd_context *c, int k, const double *rr, const double *p, double *energies) {
@@ -851,9 +821,11 @@ int bd_run_steps(bd_context *c, int k, const double *rr, const double *p, double
     HIPCHK(hipMemcpyAsync(d_rr, rr, sizeof(double) * k * n3, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(d_p, p, sizeof(double) * k * nd3, hipMemcpyHostToDevice, c->stream));
     for (int s = 0; s < k; s++) {
-        int e = stage_inputs(c, d_rr + s * n3, d_p + s * nd3, d_e + s);
-        if (!e) e = run_step(c);
+        HIPCHK(hipMemcpyAsync(c->rr, d_rr + s * n3, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->p, d_p + s * nd3, sizeof(double) * nd3, hipMemcpyDeviceToDevice, c->stream));
+        int e = run_step(c);
         if (e) return e;
+        HIPCHK(hipMemcpyAsync(d_e + s, c->energy, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     }
     if (energies) HIPCHK(hipMemcpyAsync(energies, d_e, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
     return sync_check(c);

@@ -1273,19 +1273,13 @@ __global__ void __launch_bounds__(kCCThreads) k_cc_work(Stat s, Geo g, const dou
     }
 }
 
-// Energy partials -> one double, fixed order (one workgroup); also to the
-// address held in *extra when that is set (the caller's energy slot of the
-// step, written per call by the host entry point).
-__global__ void k_ereduce(const double *Epart, int ne, double *out, double *const *extra) {
+// Energy partials -> one double, fixed order (one workgroup).
+__global__ void k_ereduce(const double *Epart, int ne, double *out) {
     __shared__ double red[16];
     double v = 0.0;
     for (int i = threadIdx.x; i < ne; i += blockDim.x) v += Epart[i];
     double t = block_sum<1024>(v, red);
-    if (threadIdx.x == 0) {
-        out[0] = t;
-        double *e = extra ? *extra : nullptr;
-        if (e) e[0] = t;
-    }
+    if (threadIdx.x == 0) out[0] = t;
 }
 
 __global__ void k_fill(double *p, long long n, double v) {
@@ -1404,8 +1398,8 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
 
 int energy_slots(const Stat &s) { return s.n + s.n_lnk + s.n_tail + s.n_LH + 2 * s.n_c; }
 
-void launch_energy_reduce(const double *Epart, int ne, double *out, hipStream_t q, double *const *extra) {
-    k_ereduce<<<1, 1024, 0, q>>>(Epart, ne, out, extra);
+void launch_energy_reduce(const double *Epart, int ne, double *out, hipStream_t q) {
+    k_ereduce<<<1, 1024, 0, q>>>(Epart, ne, out);
 }
 
 void launch_fill(double *p, long long n, double v, hipStream_t q) {

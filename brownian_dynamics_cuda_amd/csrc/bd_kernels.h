@@ -45,8 +45,7 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
                    double *Epart, int terms, hipStream_t q, MarkFn mark = nullptr, void *arg = nullptr,
                    const ForceStreams *fs = nullptr);
 int energy_slots(const Stat &s);
-void launch_energy_reduce(const double *Epart, int ne, double *out, hipStream_t q,
-                          double *const *extra = nullptr);
+void launch_energy_reduce(const double *Epart, int ne, double *out, hipStream_t q);
 void launch_fill(double *p, long long n, double v, hipStream_t q);
 void launch_fill_int(int *p, long long n, int v, hipStream_t q);
 
@@ -94,10 +93,6 @@ struct CholWork {
     // smallest trailing matrix (block rows) that takes the packed form; -1:
     // the mode's default
     int pack_min_rows;
-    // look-ahead across super-panels (all three set: on): the stream that
-    // takes the rest of each trailing update, and the events that order it
-    hipStream_t la_q;
-    hipEvent_t la_pack, la_rest;
 };
 // bytes of the packed / split panel buffer for leading dimension ld and
 // panels of nb

@@ -191,11 +191,6 @@ def test_device_noise_path_equals_host_noise_path():
         eng.synchronize()
         e2 = float(d_e.download(1)[0])
         s2 = eng.get_state()
-        # a later host-buffer step leaves the earlier call's energy slot alone
-        d_e.upload(np.array([-7.0]))
-        eng.step(rr, p)
-        eng.synchronize()
-        assert float(d_e.download(1)[0]) == -7.0
     assert e1 == e2 and np.array_equal(s1.r, s2.r) and np.array_equal(s1.r_lh, s2.r_lh)
 
 
@@ -391,30 +386,6 @@ def test_packed_panels_equal_direct(mode, panel, monkeypatch):
             eng.set_state(st, rebuild=True)
             eng.factor()
             Ls[pack], neg = eng.get_factor()
-            assert neg == 0
-    assert np.array_equal(Ls["1"], Ls["0"])
-
-
-@pytest.mark.parametrize("pack", ["packed", "direct"])
-@pytest.mark.parametrize("mode", [E.CHOL_FP64, E.CHOL_FP32, E.CHOL_BF16])
-@pytest.mark.parametrize("panel", [2, 3, 8])
-def test_lookahead_equals_serial(panel, mode, pack, monkeypatch):
-    """Look-ahead across super-panels (each trailing update split into the
-    next super-panel's columns and the rest of the trailing matrix, the rest
-    on a second stream beside the next panel's columns) against the serial
-    schedule (BD_LOOKAHEAD=0): the same tiles take the same sums, so the same
-    factor bit for bit, for the packed and the direct trailing kernels."""
-    fib, st = deck("small", hi_all=1)
-    monkeypatch.setenv("BD_PANEL_PACK", "1" if pack == "packed" else "0")
-    monkeypatch.setenv("BD_PANEL_PACK_MIN_ROWS", "0")
-    Ls = {}
-    for la in ("1", "0"):
-        monkeypatch.setenv("BD_LOOKAHEAD", la)
-        with E.BDEngine(fib, chol_mode=mode, use_graph=False) as eng:
-            eng.set_chol_schedule(panel)
-            eng.set_state(st, rebuild=True)
-            eng.factor()
-            Ls[la], neg = eng.get_factor()
             assert neg == 0
     assert np.array_equal(Ls["1"], Ls["0"])
 
