@@ -14,7 +14,9 @@ import tempfile
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liboracle.so")
+# BD_ORACLE_SO selects another build of the oracle (the sanitizer builds of
+# oracle/Makefile's sanitize target)
+ORACLE_SO = os.environ.get("BD_ORACLE_SO") or os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libbdref.so")
 
 _dp = C.POINTER(C.c_double)
