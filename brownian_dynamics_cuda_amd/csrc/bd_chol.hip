@@ -855,7 +855,10 @@ __device__ __forceinline__ void trap_index(int idx, int jcols, int &i, int &j) {
 // jcols < M restricts the update to the columns of a super-panel.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, int jcols, double *Linv, int *neg) {
-    __shared__ double W[TB * LDP], As[TB * LDP], Bs[TB * LDP], Cs[TB * LDP];
+    // three tiles of LDS: the diagonal tile's workgroup keeps its target
+    // tile in the third (the factor's T), the others the column panel block
+    // there and their target tile in registers
+    __shared__ double W[TB * LDP], As[TB * LDP], BC[TB * LDP];
     // workgroups go round-robin over the eight XCDs (b % 8); XCD x takes a
     // contiguous run of tiles, so the tiles of a block row, which read the
     // same panel tile, share an L2 (workgroup 0, the diagonal tile, stays 0)
@@ -867,39 +870,71 @@ __global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, int jco
     const int ib = kb + 1 + ii, jb = kb + 1 + jj;
     const bool diag = ib == jb;
     {
-        // inv(L_kb,kb), the panel blocks and the target tile, all in flight
+        // inv(L_kb,kb), the panel blocks (or the diagonal target tile), all in flight
         const double *const src[4] = {Linv + (size_t)kb * TB * TB, A + (size_t)ib * TB * ld + kb * TB,
-                                      A + (size_t)ib * TB * ld + jb * TB, A + (size_t)jb * TB * ld + kb * TB};
-        const int lds[4] = {TB, ld, ld, ld};
-        double *const dst[4] = {W, As, Cs, Bs};
-        load_tiles(diag ? 3 : 4, src, lds, dst);
+                                      diag ? A + (size_t)ib * TB * ld + jb * TB : A + (size_t)jb * TB * ld + kb * TB,
+                                      nullptr};
+        const int lds[4] = {TB, ld, ld, 0};
+        double *const dst[4] = {W, As, BC, nullptr};
+        load_tiles(3, src, lds, dst);
     }
     __syncthreads();
     {
         dbl4 pi[4], pj[4];
         for (int J = 0; J < 4; J++) pi[J] = pj[J] = dbl4{0.0, 0.0, 0.0, 0.0};
         mma_panel(As, W, pi);
-        if (!diag) mma_panel(Bs, W, pj);
+        if (!diag) mma_panel(BC, W, pj);
         __syncthreads();
         panel_to_lds(As, pi);
-        if (!diag) panel_to_lds(Bs, pj);
+        if (!diag) panel_to_lds(BC, pj);
     }
     __syncthreads();
     if (jj == 0) store_tile(A, ld, kb * TB, ib * TB, As, false);
-    if (MODE == 0 && diag) {
-        syrk_lower(As, Cs);
-    } else {
-        typename AccOf<MODE>::T acc[2][2];
-        for (int a = 0; a < 2; a++)
-            for (int b = 0; b < 2; b++) acc[a][b] = {};
-        tile_mma<MODE>(As, diag ? As : Bs, acc);
-        acc_to_lds(Cs, acc, -1.0, false);
+    if (diag) {
+        if (MODE == 0) {
+            syrk_lower(As, BC);
+        } else {
+            typename AccOf<MODE>::T acc[2][2];
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++) acc[a][b] = {};
+            tile_mma<MODE>(As, As, acc);
+            acc_to_lds(BC, acc, -1.0, false);
+        }
+        __syncthreads();
+        if (ii == 0)
+            diag_factor_inv(BC, W, As, A, ld, ib * TB, Linv + (size_t)ib * TB * TB, neg);
+        else
+            store_tile(A, ld, ib * TB, jb * TB, BC, true);
+        return;
     }
-    __syncthreads();
-    if (ii == 0 && jj == 0)
-        diag_factor_inv(Cs, W, As, A, ld, ib * TB, Linv + (size_t)ib * TB * TB, neg);
-    else
-        store_tile(A, ld, ib * TB, jb * TB, Cs, diag);
+    typename AccOf<MODE>::T acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = {};
+    tile_mma<MODE>(As, BC, acc);
+    // C -= acc in place from registers (the element mapping of acc_to_lds)
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32, i16 = lane & 15, k4 = lane >> 4;
+    constexpr bool f64 = MODE == 0;
+    double *C = A + (size_t)ib * TB * ld + jb * TB;
+    double cv[2][2][4];
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+        for (int ni = 0; ni < 2; ni++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = wr + 16 * mi + (f64 ? k4 + 4 * v : 4 * k4 + v), c = wc + 16 * ni + i16;
+                cv[mi][ni][v] = C[(size_t)r * ld + c];
+            }
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+        for (int ni = 0; ni < 2; ni++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = wr + 16 * mi + (f64 ? k4 + 4 * v : 4 * k4 + v), c = wc + 16 * ni + i16;
+                C[(size_t)r * ld + c] = cv[mi][ni][v] + -1.0 * (double)acc[mi][ni][v];
+            }
 }
 
 // The same block column in two launches, for wide trailing matrices (many
