@@ -855,9 +855,10 @@ __device__ __forceinline__ void trap_index(int idx, int jcols, int &i, int &j) {
 // jcols < M restricts the update to the columns of a super-panel.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_step(double *A, int ld, int kb, int jcols, double *Linv, int *neg) {
-    // three tiles of LDS: the diagonal tile's workgroup keeps its target
-    // tile in the third (the factor's T), the others the column panel block
-    // there and their target tile in registers
+    // three tiles of LDS: inv(L_kb,kb), the row's panel block, and a third
+    // holding the target tile in the diagonal tiles' workgroups (the factor's
+    // T) and the column's panel block in the others, whose target tile is
+    // updated in registers
     __shared__ double W[TB * LDP], As[TB * LDP], BC[TB * LDP];
     // workgroups go round-robin over the eight XCDs (b % 8); XCD x takes a
     // contiguous run of tiles, so the tiles of a block row, which read the
