@@ -1589,7 +1589,8 @@ __global__ void __launch_bounds__(256) k_dmul(const double *xs, const double *ra
 void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, const CholWork &c, hipStream_t q,
                 bool factor_first) {
     const int nt = c.mpad / TB;
-    if (c.xs && c.m > 0) k_hi_snapshot<<<nblk(c.m / 3, 256), 256, 0, q>>>(s, g, c.m / 3, c.xs);
+    // the positions D is built from, kept only where a refinement can read them
+    if (refine_active(c)) k_hi_snapshot<<<nblk(c.m / 3, 256), 256, 0, q>>>(s, g, c.m / 3, c.xs);
     // the first tile's factor inside the assembly launch saves a launch on
     // the factor's chain, but gives every workgroup the factor's LDS (one
     // per CU): from 64 block columns the assembly takes its own launch

@@ -100,6 +100,9 @@ struct bd_context {
     int rp_freq = 1;
     long long rp_count = 0;
     bool factor_fresh = false;
+    // chol.xs holds the positions of the last assembly of D (it is written
+    // only in the modes whose noise takes the refinement)
+    bool xs_valid = false;
     // noise and energies of bd_run_steps, grown on demand
     double *batch = nullptr;
     size_t batch_cap = 0;
@@ -188,6 +191,7 @@ void enqueue_factor(bd_context *c, const Geo &g, hipStream_t q, hipEvent_t gate 
     Marker M{c};
     M(K_RPY, 0);
     launch_rpy(c->st, c->n_hi, g, c->rad_all, c->chol, q, true);
+    c->xs_valid = refine_active(c->chol);
     M(K_RPY, 1);
     launch_cholesky(c->chol, q, M.fn(), c, gate, gate_kb);
 }
@@ -749,7 +753,9 @@ int bd_get_derived(bd_context *c, bd_derived *d) {
 }
 
 static int run_step(bd_context *c) {
-    const bool refactor = !c->factor_fresh || c->rp_count % c->rp_freq == 0;
+    // a refinement switched on since the last assembly needs a new one
+    const bool refactor = !c->factor_fresh || c->rp_count % c->rp_freq == 0 ||
+                          (refine_active(c->chol) && !c->xs_valid);
     c->rp_count++;
     c->factor_fresh = true;
     if (c->timing || !c->use_graph) {
@@ -887,6 +893,7 @@ int bd_get_tensor(bd_context *c, double *D) {
     if (!c->state_set) return fail(BD_E_STATE, "bd_set_state must be called first");
     HIPCHK(hipSetDevice(c->device));
     launch_rpy(c->st, c->n_hi, c->cur, c->rad_all, c->chol, c->stream);
+    c->xs_valid = refine_active(c->chol);
     c->factor_fresh = false;  // D now overwrites the factor's storage
     int e = sync_check(c);
     if (e) return e;
@@ -900,10 +907,14 @@ int bd_get_tensor(bd_context *c, double *D) {
 
 int bd_noise_displacement(bd_context *c, const double *p, double *rd) {
     if (check(c) || !p || !rd) return fail(BD_E_ARG, "null argument");
+    const bool refine = refine_active(c->chol);
+    if (refine && !c->xs_valid)
+        return fail(BD_E_STATE, "the factor was formed without the refinement (FP64 mode or refinement off): "
+                                "call bd_factor again");
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(c->p, p, sizeof(double) * c->n_D3, hipMemcpyHostToDevice, c->stream));
     const double *du = nullptr;
-    if (refine_active(c->chol)) {
+    if (refine) {
         launch_refine(c->st, c->chol, c->p, c->rad_all, c->stream);
         du = c->chol.pr;
     }

@@ -100,7 +100,8 @@ size_t split_bytes(int ld, int nb);
 // super-panel width the factor of c uses (0: one block column at a time)
 int chol_panel_tiles(const CholWork &c);
 // D into c.A; also clears the negative-pivot count of the factor that
-// follows, and with factor_first factors and inverts the first diagonal tile
+// follows, and with factor_first factors and inverts the first diagonal tile.
+// Where refine_active(c), the hydrodynamic-set positions go to c.xs too.
 void launch_rpy(const Stat &s, int n_hi, const Geo &g, const double *rad_all, const CholWork &c,
                 hipStream_t q, bool factor_first = false);
 // the factor of the D that launch_rpy(factor_first) left in c.A; gate

@@ -152,7 +152,10 @@ int bd_get_factor(bd_context *ctx, double *L, int *neg_pivots);
 int bd_get_tensor(bd_context *ctx, double *D);
 /* rd = s2dt * L * p for the current factor (rd_cal, func_cuda.cu:634-647);
  * in the FP32 / BF16 modes with refinement on (bd_set_refine), the refined
- * s2dt * L~ p' of the step. */
+ * s2dt * L~ p' of the step.  The refinement reads the positions D was
+ * assembled from, which an assembly keeps only while a refinement is on:
+ * BD_E_STATE when the mode or the refinement was switched on after the
+ * factor was formed (factor again). */
 int bd_noise_displacement(bd_context *ctx, const double *p, double *rd);
 int bd_set_chol_mode(bd_context *ctx, int chol_mode);
 
@@ -175,7 +178,8 @@ int bd_set_cell_list(bd_context *ctx, int enable);
  * FP64 triangular solve with the factor and one matrix-free D u product
  * from the positions D was assembled from.  enable = 1 (the default)
  * refines in the reduced-precision modes; FP64 mode never refines.  The
- * environment variable BD_REFINE sets the initial value. */
+ * environment variable BD_REFINE sets the initial value.  A step after the
+ * refinement is switched on forms D and L again even under frequency_RP. */
 int bd_set_refine(bd_context *ctx, int enable);
 
 /* Schedule of the blocked factor (no counterpart in the reference, which

@@ -119,3 +119,52 @@ def test_refined_step_graph_equals_eager():
     assert out[True][0] == out[False][0]
     for k in ("r", "a", "b", "c", "r_t", "r_lh"):
         np.testing.assert_array_equal(getattr(out[True][1], k), getattr(out[False][1], k), err_msg=k)
+
+
+def test_refine_switched_on_after_the_factor():
+    """The positions the refinement reads are kept only by an assembly made
+    while a refinement is on: switching the FP32 mode (from FP64) or the
+    refinement on after the factor asks for a new factor, which then gives
+    the refined noise of a context that had it on from the start."""
+    fib, st = Fb.make_config(2)
+    ref = _device(fib, st, E.CHOL_FP32, True)[3]
+    p = _noise(fib)
+    for start_mode, start_refine in ((E.CHOL_FP64, True), (E.CHOL_FP32, False)):
+        print(f"case: refine switched on, from mode {start_mode} refine {start_refine}", flush=True)
+        with E.BDEngine(fib, chol_mode=start_mode, use_graph=False) as eng:
+            eng.set_refine(start_refine)
+            eng.set_state(st, rebuild=True)
+            eng.factor()
+            eng.set_chol_mode(E.CHOL_FP32)
+            eng.set_refine(True)
+            with pytest.raises(E.BDError, match="bd_factor again"):
+                eng.noise_displacement(p)
+            eng.factor()
+            np.testing.assert_array_equal(eng.noise_displacement(p), ref)
+
+
+def test_step_after_refine_switched_on_forms_a_new_factor():
+    """Under frequency_RP the step after the refinement is switched on forms
+    D and L again (bit for bit a step whose factor was forced fresh by a mode
+    round trip), in the captured graph."""
+    fib, st = Fb.make_config(2)
+    g = O.Rng(6)
+    nD3 = 3 * (fib.n + fib.n_tail + fib.n_LH)
+    noise = [g.step_noise(s, fib.r_var(), nD3) for s in range(3)]
+    out = []
+    for force in (False, True):
+        with E.BDEngine(fib, chol_mode=E.CHOL_FP32, use_graph=True) as eng:
+            eng.set_refine(False)
+            eng.set_frequency_rp(5)
+            eng.set_state(st, rebuild=True)
+            for rr, p in noise[:2]:
+                eng.step(rr, p)
+            eng.set_refine(True)
+            if force:
+                eng.set_chol_mode(E.CHOL_FP64)
+                eng.set_chol_mode(E.CHOL_FP32)
+            e = eng.step(*noise[2])
+            out.append((e, eng.get_state()))
+    assert out[0][0] == out[1][0]
+    for k in ("r", "a", "b", "c", "r_t", "r_lh"):
+        np.testing.assert_array_equal(getattr(out[0][1], k), getattr(out[1][1], k), err_msg=k)
