@@ -56,6 +56,11 @@ struct Stat {
     const int *core_bead; // n_c: bead index of core k
     const int *elig;      // n: bead takes part in the 25 nm core/linker terms
     const int *lnk;       // n_lnk: bead indices of linker beads
+    // chain segments of the frame passes: nfseg + 1 bead indices, segment k
+    // = beads fseg[k] .. fseg[k+1]-1 (at most 64), each starting at bead 0 or
+    // at a core, the last ending at n - 1; nfseg = 0: one segment
+    const int *fseg;
+    int nfseg;
     const double *phi_o, *rad;
     const double *cpos, *cq;  // 3Nq, Nq
     const double *tail_pos;   // 3 n_t

@@ -394,6 +394,24 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     }
     for (int k = 0; k < n_c; k++)
         if (sys->nc_lh_flag && sys->nc_lh_flag[k] == 1) { lh_of_core[k] = (int)lhc_bead.size(); lhc_bead.push_back(core_bead[k]); }
+    // frame-pass segments: cut only in front of cores (pass 1 writes linker
+    // frames and reads across beads only core frames; pass 2 of bead i reads
+    // pass 1 of beads i and i + 1), at most 64 beads each
+    std::vector<int> fseg;
+    if (n >= 2) {
+        int cur = 0;
+        fseg.push_back(0);
+        while (n - 1 - cur > 64) {
+            int cut = -1;
+            for (int b = cur + 64; b > cur; b--)
+                if (sys->type[b] != 0) { cut = b; break; }
+            if (cut < 0) break;
+            fseg.push_back(cut);
+            cur = cut;
+        }
+        if (n - 1 - cur <= 64) fseg.push_back(n - 1);
+        else fseg.clear();
+    }
     std::vector<int> t_rcore(std::max(nt, 1), -1), rt_off(n_c + 1, 0), rt_list;
     {
         std::vector<std::vector<int>> per(n_c);
@@ -450,6 +468,8 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     UPI(core_bead, core_bead.data(), core_bead.size());
     UPI(elig, elig.data(), n);
     UPI(lnk, lnk.data(), lnk.size());
+    UPI(fseg, fseg.data(), fseg.size());
+    s.nfseg = fseg.empty() ? 0 : (int)fseg.size() - 1;
     UPD(phi_o, sys->phi_o, n);
     UPD(rad, sys->rad, n);
     UPD(cpos, sys->core_pos, 3 * Nq);

@@ -205,6 +205,25 @@ __global__ void __launch_bounds__(kFrameThreads) k_frames_betas(Stat s, Geo g, i
     if (i < s.n_tail + s.n_LH) beta_site(s, g, i, do_t, do_l);
 }
 
+// The same with the chain cut into segments that start at cores (Stat
+// fseg): a one-wavefront workgroup per segment runs pass 1 and, after a
+// barrier, pass 2 of its beads; nothing one segment writes is read by
+// another (pass 1 reads core frames only across beads, and the bead after
+// a segment is a core).  The bending angles follow, 64 sites a workgroup.
+constexpr int kSegThreads = 64;
+__global__ void __launch_bounds__(kSegThreads) k_frames_seg(Stat s, Geo g, int do_t, int do_l) {
+    if ((int)blockIdx.x < s.nfseg) {
+        const int i = s.fseg[blockIdx.x] + threadIdx.x;
+        const bool mine = i < s.fseg[blockIdx.x + 1];
+        if (mine) euler1_bead(s, g, i);
+        __syncthreads();
+        if (mine) euler2_bead(s, g, i);
+        return;
+    }
+    const int i = (blockIdx.x - s.nfseg) * kSegThreads + threadIdx.x;
+    if (i < s.n_tail + s.n_LH) beta_site(s, g, i, do_t, do_l);
+}
+
 // rotation_cal + rotate (func_cuda.cu:681-704, func.cpp:1607-1670): one
 // thread per bead, the increment of every bead formed before any frame is
 // written (the reference GPU kernel races here, func_cuda.cu:702).
@@ -365,8 +384,13 @@ void launch_frames(const Stat &s, const Geo &g, hipStream_t q) {
 }
 
 bool launch_frames_betas(const Stat &s, const Geo &g, int tails, int lh, hipStream_t q) {
-    if (s.n < 2 || s.n - 1 > kFrameThreads) return false;
+    if (s.n < 2) return false;
     const long long tot = (tails || lh) ? (long long)s.n_tail + s.n_LH : 0;
+    if (s.nfseg > 0) {
+        k_frames_seg<<<s.nfseg + nblk(tot, kSegThreads), kSegThreads, 0, q>>>(s, g, tails, lh);
+        return true;
+    }
+    if (s.n - 1 > kFrameThreads) return false;
     k_frames_betas<<<1 + nblk(tot, kFrameThreads), kFrameThreads, 0, q>>>(s, g, tails, lh);
     return true;
 }
