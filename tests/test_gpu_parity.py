@@ -51,6 +51,24 @@ def test_frame_update(kind):
         assert rel_err(d[k], fs.view(k)) <= 1e-13, k
 
 
+@pytest.mark.parametrize("n_c,linker_beads", [(3, 70), (2, 1100)])
+def test_frame_update_uncut_chain(n_c, linker_beads):
+    """Chains with more than 64 beads between cores cannot be cut into
+    frame segments: the frames then take the single-workgroup launch (up to
+    1025 beads) or the two pass launches, against the oracle as above."""
+    print(f"case: frames of an uncut chain, {n_c} cores, {linker_beads} linker beads", flush=True)
+    fib, st = Fb.make_fiber(n_c, seed=7, linker_beads=linker_beads)
+    fs = O.Oracle(fib).prepare(st)
+    with E.BDEngine(fib, use_graph=False) as eng:
+        eng.set_state(st, rebuild=True)
+        g = eng.get_state()
+        d = eng.get_derived()
+    for k in ("a", "b", "c"):
+        assert rel_err(getattr(g, k), fs.view(k)) <= 1e-14, k
+    for k in ("length", "a_dna", "b_dna", "c_dna", "beta_t", "beta_lh"):
+        assert rel_err(d[k], fs.view(k)) <= 1e-13, k
+
+
 @pytest.mark.parametrize("kind", ["tiny", "small", "cfg1"])
 @pytest.mark.parametrize("term", list(TERMS))
 def test_force_terms(kind, term):
