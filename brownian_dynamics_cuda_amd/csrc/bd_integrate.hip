@@ -305,7 +305,7 @@ __device__ __forceinline__ void rpy_block(double3 ri, double3 rj, double radi, d
 // the hydrodynamic set, threads over partners, fixed-order reduction; the
 // workgroups past n_hi take kTrThreads beads outside the set each
 // (self-mobility a1/rad only).
-constexpr int kTrThreads = 128;
+constexpr int kTrThreads = 256;
 
 __device__ __forceinline__ void species(const Stat &s, const Geo &src, const Geo &dst, const Frc &f, int b,
                                         const double *&x0, const double *&F, double *&out, int &lo, int &cnt) {
