@@ -55,7 +55,7 @@ int fail(int code, const char *fmt, ...) {
 
 const char *kKernelNames[K_COUNT] = {
     "sites", "mech", "linker", "tail", "lh", "core", "cc_list", "cc_work", "rpy",
-    "chol_step", "noise", "frames", "betas", "rotate", "translate", "average",
+    "chol_step", "noise", "frames", "betas", "translate",
     "commit", "energy_reduce", "cc_gather", "chol_trail", "chol_diag", "refine"};
 
 }  // namespace
@@ -83,7 +83,7 @@ struct bd_context {
     int nlh = 0, groups_per_core = 10;
     Stat st{};
     Geo cur{}, pred{};
-    Frc f0{}, f1{}, fa{};
+    Frc f0{}, f1{};
     Sites sites{};
     ForceScratch fw{};
     CholWork chol{};
@@ -238,13 +238,14 @@ void enqueue_step(bd_context *c, bool refactor) {
     // force pass 1 at x0 and its energy (3556-3615)
     enqueue_forces(c, c->cur, c->f0, c->Epart, BD_TERM_ALL, 0);
     M(K_EREDUCE, 0); launch_energy_reduce(c->Epart, c->ne, c->energy, q); M(K_EREDUCE, 1);
-    // predictor (3617-3619)
-    M(K_ROTATE, 0); launch_rotate(s, c->cur, c->pred, c->f0.T, c->rr, q); M(K_ROTATE, 1);
     if (qf != q) {
         (void)hipEventRecord(c->ev_join, qf);
         (void)hipStreamWaitEvent(q, c->ev_join, 0);
     }
-    M(K_TRANSLATE, 0); launch_translate(s, c->hi_all, c->cur, c->pred, c->f0, c->rd, c->rad_all, q); M(K_TRANSLATE, 1);
+    // predictor (3617-3619): translation and rotation in one launch
+    M(K_TRANSLATE, 0);
+    launch_translate(s, c->hi_all, c->cur, c->pred, c->f0, Frc{}, c->rd, c->rad_all, c->rr, q);
+    M(K_TRANSLATE, 1);
     // rebuild at the predicted state (3621-3625)
     M(K_FRAMES, 0);
     if (!launch_frames_betas(s, c->pred, 1, 1, q)) {
@@ -255,8 +256,10 @@ void enqueue_step(bd_context *c, bool refactor) {
     // force pass 2 (3628-3646)
     enqueue_forces(c, c->pred, c->f1, nullptr, BD_TERM_ALL, 1);
     // corrector from x0 with the averaged forces, same rd and rr (3648-3652)
-    M(K_AVG, 0); launch_average_rotate(s, c->f0, c->f1, c->fa, c->cur, c->pred, c->rr, q); M(K_AVG, 1);
-    M(K_TRANSLATE, 0); launch_translate(s, c->hi_all, c->cur, c->pred, c->fa, c->rd, c->rad_all, q); M(K_TRANSLATE, 1);
+    // (the Heun means of forces and torques formed where they are read)
+    M(K_TRANSLATE, 0);
+    launch_translate(s, c->hi_all, c->cur, c->pred, c->f0, c->f1, c->rd, c->rad_all, c->rr, q);
+    M(K_TRANSLATE, 1);
     // commit and rebuild (3654-3660)
     M(K_COMMIT, 0); launch_commit(s, c->pred, c->cur, q); M(K_COMMIT, 1);
     M(K_FRAMES, 0);
@@ -506,7 +509,7 @@ int bd_create(const bd_system *sys, const bd_options *opt, bd_context **out) {
     if (r) return bail(BD_E_HIP);
 
     if (alloc_geo(c, c->cur) || alloc_geo(c, c->pred)) return bail(BD_E_HIP);
-    if (alloc_frc(c, c->f0) || alloc_frc(c, c->f1) || alloc_frc(c, c->fa)) return bail(BD_E_HIP);
+    if (alloc_frc(c, c->f0) || alloc_frc(c, c->f1)) return bail(BD_E_HIP);
     r |= c->alloc(&c->sites.Q, 3 * (size_t)n_c * Nq);
     r |= c->alloc(&c->sites.Qp, 4 * (size_t)n_c * Nq);
     r |= c->alloc(&c->sites.G, 3 * (size_t)n_c * std::max(sys->n_lh_g, 1));

@@ -264,12 +264,6 @@ __device__ __forceinline__ void rotate_bead(const Stat &s, const Geo &src, const
         cn[k] = gc0 * A[k] + gc1 * B[k] + gc2 * C[k];
     }
 }
-__global__ void k_rotate(Stat s, Geo src, Geo dst, const double *T, const double *rr) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= s.n) return;
-    const double Tj[3] = {T[3 * j], T[3 * j + 1], T[3 * j + 2]};
-    rotate_bead(s, src, dst, j, Tj, rr);
-}
 
 // RPY 3x3 block between beads at ri and rj (func.cpp:752-799); rows are
 // the components of bead i.  Identical value for (j, i): only products of
@@ -307,36 +301,53 @@ __device__ __forceinline__ void rpy_block(double3 ri, double3 rj, double radi, d
 // (self-mobility a1/rad only).
 constexpr int kTrThreads = 256;
 
-__device__ __forceinline__ void species(const Stat &s, const Geo &src, const Geo &dst, const Frc &f, int b,
-                                        const double *&x0, const double *&F, double *&out, int &lo, int &cnt) {
-    if (b < s.n) { x0 = src.r; F = f.F; out = dst.r; lo = 0; cnt = s.n; }
-    else if (b < s.n + s.n_tail) { x0 = src.r_t; F = f.tF; out = dst.r_t; lo = s.n; cnt = s.n_tail; }
-    else { x0 = src.r_lh; F = f.lF; out = dst.r_lh; lo = s.n + s.n_tail; cnt = s.n_LH; }
+__device__ __forceinline__ void species(const Stat &s, const Geo &src, const Geo &dst, const Frc &f, const Frc &f2,
+                                        int b, const double *&x0, const double *&F, const double *&F2,
+                                        double *&out, int &lo, int &cnt) {
+    if (b < s.n) { x0 = src.r; F = f.F; F2 = f2.F; out = dst.r; lo = 0; cnt = s.n; }
+    else if (b < s.n + s.n_tail) { x0 = src.r_t; F = f.tF; F2 = f2.tF; out = dst.r_t; lo = s.n; cnt = s.n_tail; }
+    else { x0 = src.r_lh; F = f.lF; F2 = f2.lF; out = dst.r_lh; lo = s.n + s.n_tail; cnt = s.n_LH; }
+}
+// force component i: F's, or the Heun mean of F and F2 (force_torque_tmp,
+// func_cuda.cu:2626-2641) formed where it is read
+__device__ __forceinline__ double fcomp(const double *F, const double *F2, int i) {
+    return F2 ? 0.5 * (F[i] + F2[i]) : F[i];
 }
 
-__global__ void __launch_bounds__(kTrThreads) k_translate(Stat s, int n_hi, Geo src, Geo dst, Frc f,
-                                                         const double *rd, const double *rad_all) {
+__global__ void __launch_bounds__(kTrThreads) k_translate(Stat s, int n_hi, int ntr, Geo src, Geo dst, Frc f,
+                                                         Frc f2, const double *rd, const double *rad_all,
+                                                         const double *rr) {
     __shared__ double red[3 * (kTrThreads / 64)];
     const int b = blockIdx.x;
-    const double *x0, *F;
+    const double *x0, *F, *F2;
     double *out;
     int lo, cnt;
+    if (b >= ntr) {
+        // the rotation of the main beads (frames only; the translation
+        // writes positions only)
+        const int j = (b - ntr) * kTrThreads + threadIdx.x;
+        if (j >= s.n) return;
+        const double Tj[3] = {fcomp(f.T, f2.T, 3 * j), fcomp(f.T, f2.T, 3 * j + 1), fcomp(f.T, f2.T, 3 * j + 2)};
+        rotate_bead(s, src, dst, j, Tj, rr);
+        return;
+    }
     if (b >= n_hi) {
         const int bd = n_hi + (b - n_hi) * kTrThreads + threadIdx.x;
         if (bd >= s.n + s.n_tail + s.n_LH) return;
-        species(s, src, dst, f, bd, x0, F, out, lo, cnt);
+        species(s, src, dst, f, f2, bd, x0, F, F2, out, lo, cnt);
         const int i = bd - lo;
         const double dself = s.a1 / rad_all[bd];
-        for (int k = 0; k < 3; k++) out[3 * i + k] = x0[3 * i + k] + rd[3 * bd + k] + s.del * dself * F[3 * i + k];
+        for (int k = 0; k < 3; k++)
+            out[3 * i + k] = x0[3 * i + k] + rd[3 * bd + k] + s.del * dself * fcomp(F, F2, 3 * i + k);
         return;
     }
-    species(s, src, dst, f, b, x0, F, out, lo, cnt);
+    species(s, src, dst, f, f2, b, x0, F, F2, out, lo, cnt);
     const int i = b - lo;
     const double radi = rad_all[b];
     const double3 ri = ld3(x0 + 3 * i);
     double v[3] = {0.0, 0.0, 0.0};
     for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
-        const double *Fj = F + 3 * j;
+        const double Fj[3] = {fcomp(F, F2, 3 * j), fcomp(F, F2, 3 * j + 1), fcomp(F, F2, 3 * j + 2)};
         if (j == i) {
             const double dself = s.a1 / radi;
             for (int k = 0; k < 3; k++) v[k] += s.del * dself * Fj[k];
@@ -350,23 +361,6 @@ __global__ void __launch_bounds__(kTrThreads) k_translate(Stat s, int n_hi, Geo 
     block_sum_n<kTrThreads, 3>(v, red);
     if (threadIdx.x == 0)
         for (int k = 0; k < 3; k++) out[3 * i + k] = (x0[3 * i + k] + rd[3 * b + k]) + v[k];
-}
-
-// force_torque_tmp (func_cuda.cu:2626-2641) and the corrector rotation
-// from x0 with the averaged torque: thread i averages element i of every
-// force array and rotates bead i (its torque averaged in place, the same
-// operations as the stored average).
-__global__ void k_average_rotate(Stat s, int n3, int nt3, int nl3, Frc a, Frc b, Frc o, Geo src, Geo dst,
-                                 const double *rr) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n3) { o.F[i] = 0.5 * (a.F[i] + b.F[i]); o.T[i] = 0.5 * (a.T[i] + b.T[i]); }
-    if (i < nt3) o.tF[i] = 0.5 * (a.tF[i] + b.tF[i]);
-    if (i < nl3) o.lF[i] = 0.5 * (a.lF[i] + b.lF[i]);
-    if (i < s.n) {
-        const double Tj[3] = {0.5 * (a.T[3 * i] + b.T[3 * i]), 0.5 * (a.T[3 * i + 1] + b.T[3 * i + 1]),
-                              0.5 * (a.T[3 * i + 2] + b.T[3 * i + 2])};
-        rotate_bead(s, src, dst, i, Tj, rr);
-    }
 }
 
 // final_updates (func_cuda.cu:2643-2662)
@@ -400,24 +394,12 @@ void launch_betas(const Stat &s, const Geo &g, int tails, int lh, hipStream_t q)
     if (tot > 0 && (tails || lh)) k_betas<<<nblk(tot, 128), 128, 0, q>>>(s, g, tails, lh);
 }
 
-void launch_rotate(const Stat &s, const Geo &src, const Geo &dst, const double *T, const double *rr,
-                   hipStream_t q) {
-    k_rotate<<<nblk(s.n, 64), 64, 0, q>>>(s, src, dst, T, rr);
-}
-
-void launch_translate(const Stat &s, int hi_all, const Geo &src, const Geo &dst, const Frc &f,
-                      const double *rd, const double *rad_all, hipStream_t q) {
+void launch_translate(const Stat &s, int hi_all, const Geo &src, const Geo &dst, const Frc &f, const Frc &f2,
+                      const double *rd, const double *rad_all, const double *rr, hipStream_t q) {
     const int tot = s.n + s.n_tail + s.n_LH;
     const int n_hi = hi_all ? tot : s.n;
-    k_translate<<<n_hi + nblk(tot - n_hi, kTrThreads), kTrThreads, 0, q>>>(s, n_hi, src, dst, f, rd, rad_all);
-}
-
-void launch_average_rotate(const Stat &s, const Frc &a, const Frc &b, const Frc &out, const Geo &src, const Geo &dst,
-                           const double *rr, hipStream_t q) {
-    int mx = 3 * s.n;
-    if (3 * s.n_tail > mx) mx = 3 * s.n_tail;
-    if (3 * s.n_LH > mx) mx = 3 * s.n_LH;
-    k_average_rotate<<<nblk(mx, 256), 256, 0, q>>>(s, 3 * s.n, 3 * s.n_tail, 3 * s.n_LH, a, b, out, src, dst, rr);
+    const int ntr = n_hi + nblk(tot - n_hi, kTrThreads), nrot = rr ? nblk(s.n, kTrThreads) : 0;
+    k_translate<<<ntr + nrot, kTrThreads, 0, q>>>(s, n_hi, ntr, src, dst, f, f2, rd, rad_all, rr);
 }
 
 void launch_commit(const Stat &s, const Geo &from, const Geo &to, hipStream_t q) {

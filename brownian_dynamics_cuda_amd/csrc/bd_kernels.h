@@ -30,8 +30,8 @@ void launch_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, h
 // mark(arg, id, phase): optional timing hook called around each kernel
 // (phase 0 before, 1 after); ids are the K_* values below.
 enum KernelId { K_SITES, K_MECH, K_LINKER, K_TAIL, K_LH, K_CORE, K_CCLIST, K_CCWORK,
-                K_RPY, K_CHOL_STEP, K_NOISE, K_FRAMES, K_BETAS, K_ROTATE, K_TRANSLATE,
-                K_AVG, K_COMMIT, K_EREDUCE, K_CCGATHER, K_CHOL_TRAIL, K_CHOL_DIAG, K_REFINE, K_COUNT };
+                K_RPY, K_CHOL_STEP, K_NOISE, K_FRAMES, K_BETAS, K_TRANSLATE,
+                K_COMMIT, K_EREDUCE, K_CCGATHER, K_CHOL_TRAIL, K_CHOL_DIAG, K_REFINE, K_COUNT };
 typedef void (*MarkFn)(void *, int, int);
 // Extra streams of a force pass: after the charge sites, the tail, the
 // linker-histone (then, once mechanics is done, linker) and the core-core
@@ -52,17 +52,14 @@ void launch_fill_int(int *p, long long n, int v, hipStream_t q);
 // frames and integrator (bd_integrate.hip)
 void launch_frames(const Stat &s, const Geo &g, hipStream_t q);         // update_Euler_Angle
 void launch_betas(const Stat &s, const Geo &g, int tails, int lh, hipStream_t q);
-// both, as one launch, for chains of up to 1025 beads; false (nothing
-// launched) for longer ones
+// both, as one launch, where the chain cuts into frame segments or has at
+// most 1025 beads; false (nothing launched) otherwise
 bool launch_frames_betas(const Stat &s, const Geo &g, int tails, int lh, hipStream_t q);
-void launch_rotate(const Stat &s, const Geo &src, const Geo &dst, const double *T, const double *rr,
-                   hipStream_t q);
-void launch_translate(const Stat &s, int hi_all, const Geo &src, const Geo &dst, const Frc &f,
-                      const double *rd, const double *rad_all, hipStream_t q);
-// out = (a + b) / 2, and the rotation of every bead of src into dst with
-// the averaged torque
-void launch_average_rotate(const Stat &s, const Frc &a, const Frc &b, const Frc &out, const Geo &src, const Geo &dst,
-                           const double *rr, hipStream_t q);
+// the translation from src to dst with forces f (f2.F set: the mean of f and
+// f2) and, with rr set, the rotation of the main beads' frames in the same
+// launch (torques likewise)
+void launch_translate(const Stat &s, int hi_all, const Geo &src, const Geo &dst, const Frc &f, const Frc &f2,
+                      const double *rd, const double *rad_all, const double *rr, hipStream_t q);
 void launch_commit(const Stat &s, const Geo &from, const Geo &to, hipStream_t q);
 
 // diffusion tensor, Cholesky, noise (bd_chol.hip)
