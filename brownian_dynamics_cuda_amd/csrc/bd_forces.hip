@@ -18,26 +18,11 @@ namespace bd {
 // ---------------------------------------------------------------------------
 // world positions of core charges and globular-head sites
 // ---------------------------------------------------------------------------
-__global__ void k_sites(Stat s, Geo g, Sites st, int *cc_count) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) *cc_count = 0;  // the core-core pair list of this pass starts empty
-    int nq = s.n_c * s.Nq;
-    if (t < nq) {
-        int k = t / s.Nq, q = t % s.Nq, cb = s.core_bead[k];
-        double3 w = body_to_world(ld3(g.r + 3 * cb), ld3(g.a + 3 * cb), ld3(g.b + 3 * cb),
-                                  ld3(g.c + 3 * cb), ld3(s.cpos + 3 * q));
-        st3(st.Q + 3 * t, w);
-        double *p = st.Qp + 4 * (size_t)t;
-        p[0] = w.x; p[1] = w.y; p[2] = w.z; p[3] = s.cq[q];
-        return;
-    }
-    t -= nq;
-    if (t < s.n_lhc * s.n_lh_g) {
-        int k = t / s.n_lh_g, m = t % s.n_lh_g, cb = s.lhc_bead[k];
-        double3 w = body_to_world(ld3(g.r + 3 * cb), ld3(g.a + 3 * cb), ld3(g.b + 3 * cb),
-                                  ld3(g.c + 3 * cb), ld3(s.LH_g_pos + 3 * m));
-        st3(st.G + 3 * t, w);
-    }
+// head site t (globular-domain site m of LH-bearing core k) in the lab frame
+__device__ __forceinline__ double3 head_site(const Stat &s, const Geo &g, int t) {
+    const int k = t / s.n_lh_g, m = t % s.n_lh_g, cb = s.lhc_bead[k];
+    return body_to_world(ld3(g.r + 3 * cb), ld3(g.a + 3 * cb), ld3(g.b + 3 * cb), ld3(g.c + 3 * cb),
+                         ld3(s.LH_g_pos + 3 * m));
 }
 
 // ---------------------------------------------------------------------------
@@ -140,11 +125,8 @@ __device__ __forceinline__ double3 torque_abs(double3 f, double3 pf, double3 pc,
 // the largest distance from it), by one wavefront, a member per lane; idx
 // maps a partner to its bead (NULL: the partner is the bead).  Returns the
 // sphere in every lane.
-__device__ __forceinline__ double4 chunk_sphere(const double *x, const int *idx, int c, int n) {
-    const int lane = threadIdx.x & 63, k = c * kChunk + lane;
-    const int cnt = min(n - c * kChunk, kChunk);
-    const bool in = lane < cnt;
-    const double3 p = in ? ld3(x + 3 * (idx ? idx[k] : k)) : zero3();
+// sphere of the cnt points p held by lanes 0 .. cnt-1 of a wavefront
+__device__ __forceinline__ double4 sphere_of(double3 p, bool in, int cnt) {
     double3 m = p;
     for (int o = 32; o > 0; o >>= 1) {
         m.x += __shfl_xor(m.x, o, 64);
@@ -157,6 +139,13 @@ __device__ __forceinline__ double4 chunk_sphere(const double *x, const int *idx,
     double r2 = in ? dot(d, d) : 0.0;
     for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
     return make_double4(m.x, m.y, m.z, sqrt(r2));
+}
+__device__ __forceinline__ double4 chunk_sphere(const double *x, const int *idx, int c, int n) {
+    const int lane = threadIdx.x & 63, k = c * kChunk + lane;
+    const int cnt = min(n - c * kChunk, kChunk);
+    const bool in = lane < cnt;
+    const double3 p = in ? ld3(x + 3 * (idx ? idx[k] : k)) : zero3();
+    return sphere_of(p, in, cnt);
 }
 
 // Chunk spheres of the tail beads, free LH beads, head sites and linker
@@ -172,6 +161,47 @@ __global__ void __launch_bounds__(256) k_chunks(Stat s, Geo g, Sites st) {
     if (t < c) { const double4 h = chunk_sphere(st.G, nullptr, t, s.n_lhc * s.n_lh_g); if (w0) st.ch_g[t] = h; return; }
     t -= c;
     if (t < nchunks(s.n_lnk)) { const double4 h = chunk_sphere(g.r, s.lnk, t, s.n_lnk); if (w0) st.ch_l[t] = h; }
+}
+
+// Charge sites Q (with charges, Qp) of every core and head sites G of every
+// LH-bearing core in the lab frame, a thread per site; with chunks, the
+// workgroups from nsb on form the chunk spheres as k_chunks does, a head
+// chunk from its sites formed here again (the same values k_sites writes).
+__global__ void __launch_bounds__(256) k_sites(Stat s, Geo g, Sites st, int *cc_count, int nsb) {
+    if ((int)blockIdx.x >= nsb) {
+        int t = (blockIdx.x - nsb) * 4 + (threadIdx.x >> 6);
+        const bool w0 = (threadIdx.x & 63) == 0;
+        const int a = nchunks(s.n_tail), b = nchunks(s.n_LH), c = nchunks(s.n_lhc * s.n_lh_g);
+        if (t < a) { const double4 h = chunk_sphere(g.r_t, nullptr, t, s.n_tail); if (w0) st.ch_t[t] = h; return; }
+        t -= a;
+        if (t < b) { const double4 h = chunk_sphere(g.r_lh, nullptr, t, s.n_LH); if (w0) st.ch_lh[t] = h; return; }
+        t -= b;
+        if (t < c) {
+            const int nh = s.n_lhc * s.n_lh_g, lane = threadIdx.x & 63, k = t * kChunk + lane;
+            const int cnt = min(nh - t * kChunk, kChunk);
+            const bool in = lane < cnt;
+            const double4 h = sphere_of(in ? head_site(s, g, k) : zero3(), in, cnt);
+            if (w0) st.ch_g[t] = h;
+            return;
+        }
+        t -= c;
+        if (t < nchunks(s.n_lnk)) { const double4 h = chunk_sphere(g.r, s.lnk, t, s.n_lnk); if (w0) st.ch_l[t] = h; }
+        return;
+    }
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *cc_count = 0;  // the core-core pair list of this pass starts empty
+    int nq = s.n_c * s.Nq;
+    if (t < nq) {
+        int k = t / s.Nq, q = t % s.Nq, cb = s.core_bead[k];
+        double3 w = body_to_world(ld3(g.r + 3 * cb), ld3(g.a + 3 * cb), ld3(g.b + 3 * cb),
+                                  ld3(g.c + 3 * cb), ld3(s.cpos + 3 * q));
+        st3(st.Q + 3 * t, w);
+        double *p = st.Qp + 4 * (size_t)t;
+        p[0] = w.x; p[1] = w.y; p[2] = w.z; p[3] = s.cq[q];
+        return;
+    }
+    t -= nq;
+    if (t < s.n_lhc * s.n_lh_g) st3(st.G + 3 * t, head_site(s, g, t));
 }
 
 // Ordered stream compaction over an NT-thread workgroup: threads with keep
@@ -1296,10 +1326,21 @@ __global__ void k_fill_i(int *p, long long n, int v) {
 // ---------------------------------------------------------------------------
 static inline int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
 
-void launch_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, hipStream_t q) {
+void launch_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, hipStream_t q, bool fuse_chunks) {
     long long tot = (long long)s.n_c * s.Nq + (long long)s.n_lhc * s.n_lh_g;
-    k_sites<<<nblk(std::max(tot, 1LL), 256), 256, 0, q>>>(s, g, st, cc_count);
-    if (st.use_cells) {
+    const int nsb = nblk(std::max(tot, 1LL), 256);
+    const int nch = nchunks(s.n_tail) + nchunks(s.n_LH) + nchunks(s.n_lhc * s.n_lh_g) + nchunks(s.n_lnk);
+    if (!st.use_cells && fuse_chunks) {
+        // the chunk spheres in the same launch
+        k_sites<<<nsb + nblk(nch, 4), 256, 0, q>>>(s, g, st, cc_count, nsb);
+        return;
+    }
+    k_sites<<<nsb, 256, 0, q>>>(s, g, st, cc_count, nsb);
+    if (!st.use_cells) {
+        if (nch > 0) k_chunks<<<nblk(nch, 4), 256, 0, q>>>(s, g, st);
+        return;
+    }
+    {
         // the grids are built from this pass's positions
         Sites sp = st;
         sp.cg[SET_TAIL].pos = g.r_t;
@@ -1316,10 +1357,7 @@ void launch_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, h
         sp.cg[SET_LINKER].n = s.n_lnk;
         k_cells<<<NSETS, kCellThreads, 0, q>>>(sp);
         k_cell_sort<<<512, 256, 0, q>>>(s, st);
-        return;
     }
-    const int nch = nchunks(s.n_tail) + nchunks(s.n_LH) + nchunks(s.n_lhc * s.n_lh_g) + nchunks(s.n_lnk);
-    if (nch > 0) k_chunks<<<nblk(nch, 4), 256, 0, q>>>(s, g, st);
 }
 
 void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, const ForceScratch &w,
@@ -1363,7 +1401,10 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
                                                                           E(off_core), terms, cs[0], cs[1]);
         M(K_CORE, 1);
     };
-    M(K_SITES, 0); launch_sites(s, g, st, w.cc_count, q); M(K_SITES, 1);
+    // with branches (force pass 2, after the factor has joined) the chunk
+    // spheres take the sites launch; in pass 1 beside the factor the extra
+    // launch measured faster
+    M(K_SITES, 0); launch_sites(s, g, st, w.cc_count, q, fs != nullptr); M(K_SITES, 1);
     // The enqueue orders were measured (cfg2): on one stream, beside the
     // factorisation, tail and LH first and core-core last; with branches,
     // core-core first and tail and LH last (enqueued first there, their

@@ -24,9 +24,10 @@ struct ForceScratch {
 };
 
 // forces (bd_forces.hip)
-// world charge and head sites, partner chunk spheres; resets the core-core
-// pair count of the pass
-void launch_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, hipStream_t q);
+// world charge and head sites, partner chunk spheres (with fuse_chunks in
+// the sites launch itself); resets the core-core pair count of the pass
+void launch_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, hipStream_t q,
+                  bool fuse_chunks = false);
 // mark(arg, id, phase): optional timing hook called around each kernel
 // (phase 0 before, 1 after); ids are the K_* values below.
 enum KernelId { K_SITES, K_MECH, K_LINKER, K_TAIL, K_LH, K_CORE, K_CCLIST, K_CCWORK,
