@@ -182,9 +182,10 @@ struct Marker {
 // Pass 2 (pass == 1) runs its branches on their own streams unless the
 // timing markers are on; pass 1 stays on one stream, because it already
 // shares the GPU with the factorisation on the side stream.
-void enqueue_forces(bd_context *c, const Geo &g, const Frc &f, double *Epart, int terms, int pass) {
+void enqueue_forces(bd_context *c, const Geo &g, const Frc &f, double *Epart, int terms, int pass,
+                    bool sites_done = false) {
     launch_forces(c->st, g, c->sites, f, c->fw, Epart, terms, c->stream, c->timing ? mark_cb : nullptr, c,
-                  (c->timing || pass == 0) ? nullptr : &c->fstr[pass]);
+                  (c->timing || pass == 0) ? nullptr : &c->fstr[pass], sites_done);
 }
 
 void enqueue_factor(bd_context *c, const Geo &g, hipStream_t q, hipEvent_t gate = nullptr, int gate_kb = -1) {
@@ -246,15 +247,17 @@ void enqueue_step(bd_context *c, bool refactor) {
     M(K_TRANSLATE, 0);
     launch_translate(s, c->hi_all, c->cur, c->pred, c->f0, Frc{}, c->rd, c->rad_all, c->rr, q);
     M(K_TRANSLATE, 1);
-    // rebuild at the predicted state (3621-3625)
+    // rebuild at the predicted state (3621-3625), with force pass 2's sites
+    // in the same launch (outside the per-kernel timing mode)
     M(K_FRAMES, 0);
-    if (!launch_frames_betas(s, c->pred, 1, 1, q)) {
+    const bool sites_done = !c->timing && launch_frames_sites(s, c->pred, c->sites, c->fw.cc_count, q);
+    if (!sites_done && !launch_frames_betas(s, c->pred, 1, 1, q)) {
         launch_betas(s, c->pred, 1, 1, q);
         launch_frames(s, c->pred, q);
     }
     M(K_FRAMES, 1);
     // force pass 2 (3628-3646)
-    enqueue_forces(c, c->pred, c->f1, nullptr, BD_TERM_ALL, 1);
+    enqueue_forces(c, c->pred, c->f1, nullptr, BD_TERM_ALL, 1, sites_done);
     // corrector from x0 with the averaged forces, same rd and rr (3648-3652)
     // (the Heun means of forces and torques formed where they are read)
     M(K_TRANSLATE, 0);

@@ -12,6 +12,7 @@
 // (see the comment at each term), into per-thread slots of Epart.
 #include "bd_dev.h"
 #include "bd_kernels.h"
+#include "bd_frames.h"
 
 namespace bd {
 
@@ -167,9 +168,10 @@ __global__ void __launch_bounds__(256) k_chunks(Stat s, Geo g, Sites st) {
 // LH-bearing core in the lab frame, a thread per site; with chunks, the
 // workgroups from nsb on form the chunk spheres as k_chunks does, a head
 // chunk from its sites formed here again (the same values k_sites writes).
-__global__ void __launch_bounds__(256) k_sites(Stat s, Geo g, Sites st, int *cc_count, int nsb) {
-    if ((int)blockIdx.x >= nsb) {
-        int t = (blockIdx.x - nsb) * 4 + (threadIdx.x >> 6);
+__device__ __forceinline__ void sites_block(const Stat &s, const Geo &g, const Sites &st, int *cc_count, int bid,
+                                            int nsb) {
+    if (bid >= nsb) {
+        int t = (bid - nsb) * 4 + (threadIdx.x >> 6);
         const bool w0 = (threadIdx.x & 63) == 0;
         const int a = nchunks(s.n_tail), b = nchunks(s.n_LH), c = nchunks(s.n_lhc * s.n_lh_g);
         if (t < a) { const double4 h = chunk_sphere(g.r_t, nullptr, t, s.n_tail); if (w0) st.ch_t[t] = h; return; }
@@ -188,7 +190,7 @@ __global__ void __launch_bounds__(256) k_sites(Stat s, Geo g, Sites st, int *cc_
         if (t < nchunks(s.n_lnk)) { const double4 h = chunk_sphere(g.r, s.lnk, t, s.n_lnk); if (w0) st.ch_l[t] = h; }
         return;
     }
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = bid * 256 + threadIdx.x;
     if (t == 0) *cc_count = 0;  // the core-core pair list of this pass starts empty
     int nq = s.n_c * s.Nq;
     if (t < nq) {
@@ -202,6 +204,31 @@ __global__ void __launch_bounds__(256) k_sites(Stat s, Geo g, Sites st, int *cc_
     }
     t -= nq;
     if (t < s.n_lhc * s.n_lh_g) st3(st.G + 3 * t, head_site(s, g, t));
+}
+__global__ void __launch_bounds__(256) k_sites(Stat s, Geo g, Sites st, int *cc_count, int nsb) {
+    sites_block(s, g, st, cc_count, blockIdx.x, nsb);
+}
+// The predicted state's frame passes and tail / LH bending angles (the
+// chain segments of k_frames_seg, on the first threads of a workgroup) and,
+// in the same launch, force pass 2's charge and head sites and chunk
+// spheres: the sites read positions and core frames only, which the frame
+// passes do not write.
+__global__ void __launch_bounds__(256) k_frames_sites(Stat s, Geo g, Sites st, int *cc_count, int nbeta, int nsb) {
+    const int b = blockIdx.x;
+    if (b < s.nfseg) {
+        const int i = s.fseg[b] + threadIdx.x;
+        const bool mine = i < s.fseg[b + 1];
+        if (mine) euler1_bead(s, g, i);
+        __syncthreads();
+        if (mine) euler2_bead(s, g, i);
+        return;
+    }
+    if (b < s.nfseg + nbeta) {
+        const int i = (b - s.nfseg) * 256 + threadIdx.x;
+        if (i < s.n_tail + s.n_LH) beta_site(s, g, i, 1, 1);
+        return;
+    }
+    sites_block(s, g, st, cc_count, b - s.nfseg - nbeta, nsb);
 }
 
 // Ordered stream compaction over an NT-thread workgroup: threads with keep
@@ -1360,8 +1387,18 @@ void launch_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, h
     }
 }
 
+bool launch_frames_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, hipStream_t q) {
+    if (s.nfseg == 0 || st.use_cells) return false;
+    const long long tot = (long long)s.n_c * s.Nq + (long long)s.n_lhc * s.n_lh_g;
+    const int nsb = nblk(std::max(tot, 1LL), 256), nbeta = nblk((long long)s.n_tail + s.n_LH, 256);
+    const int nch = nchunks(s.n_tail) + nchunks(s.n_LH) + nchunks(s.n_lhc * s.n_lh_g) + nchunks(s.n_lnk);
+    k_frames_sites<<<s.nfseg + nbeta + nsb + nblk(nch, 4), 256, 0, q>>>(s, g, st, cc_count, nbeta, nsb);
+    return true;
+}
+
 void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, const ForceScratch &w,
-                   double *Epart, int terms, hipStream_t q, MarkFn mark, void *arg, const ForceStreams *fs) {
+                   double *Epart, int terms, hipStream_t q, MarkFn mark, void *arg, const ForceStreams *fs,
+                   bool sites_done) {
     const int off_mech = 0, off_lnk = s.n, off_tail = off_lnk + s.n_lnk, off_lh = off_tail + s.n_tail,
               off_core = off_lh + s.n_LH, off_cc = off_core + s.n_c;
     auto E = [&](int off) { return Epart ? Epart + off : nullptr; };
@@ -1404,7 +1441,7 @@ void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, c
     // with branches (force pass 2, after the factor has joined) the chunk
     // spheres take the sites launch; in pass 1 beside the factor the extra
     // launch measured faster
-    M(K_SITES, 0); launch_sites(s, g, st, w.cc_count, q, fs != nullptr); M(K_SITES, 1);
+    if (!sites_done) { M(K_SITES, 0); launch_sites(s, g, st, w.cc_count, q, fs != nullptr); M(K_SITES, 1); }
     // The enqueue orders were measured (cfg2): on one stream, beside the
     // factorisation, tail and LH first and core-core last; with branches,
     // core-core first and tail and LH last (enqueued first there, their

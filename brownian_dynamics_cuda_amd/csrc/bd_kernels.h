@@ -44,7 +44,11 @@ struct ForceStreams {
 };
 void launch_forces(const Stat &s, const Geo &g, const Sites &st, const Frc &f, const ForceScratch &w,
                    double *Epart, int terms, hipStream_t q, MarkFn mark = nullptr, void *arg = nullptr,
-                   const ForceStreams *fs = nullptr);
+                   const ForceStreams *fs = nullptr, bool sites_done = false);
+// the predicted state's frame passes and bending angles with force pass 2's
+// sites and chunk spheres in one launch (false, nothing launched: the chain
+// has no frame segments or the partner search uses cell grids)
+bool launch_frames_sites(const Stat &s, const Geo &g, const Sites &st, int *cc_count, hipStream_t q);
 int energy_slots(const Stat &s);
 void launch_energy_reduce(const double *Epart, int ne, double *out, hipStream_t q);
 void launch_fill(double *p, long long n, double v, hipStream_t q);
