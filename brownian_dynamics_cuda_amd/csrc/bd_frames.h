@@ -1,6 +1,11 @@
 // bd_frames.h -- the per-bead bodies of the frame passes (update_Euler_Angle)
 // and of the tail / LH bending angles, shared by the frame kernels and the
 // predicted state's fused frame-and-sites launch.
+//
+// They keep the reference's operation order with fused multiply-add
+// contraction switched off: the Euler angles come out of acos() of dot
+// products that sit at +-1 for straight linkers, where any rounding change
+// is amplified to ~1e-8.
 #pragma once
 #include "bd_dev.h"
 

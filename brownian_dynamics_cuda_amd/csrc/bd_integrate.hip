@@ -1,10 +1,6 @@
 // bd_integrate.hip -- frames, Euler angles, rotation, translation, Heun
-// average and commit of the BD step (gfx950).
-//
-// Frame and angle kernels keep the reference's operation order with fused
-// multiply-add contraction switched off: the Euler angles come out of
-// acos() of dot products that sit at +-1 for straight linkers, where any
-// rounding change is amplified to ~1e-8.
+// average and commit of the BD step (gfx950).  The per-bead frame and
+// bending-angle bodies are in bd_frames.h.
 #include "bd_dev.h"
 #include "bd_kernels.h"
 #include "bd_frames.h"
