@@ -193,6 +193,31 @@ def test_graph_replay_equals_eager_and_is_deterministic():
             assert np.array_equal(getattr(s, k), getattr(outs[0][1], k)), k
 
 
+@pytest.mark.parametrize("kind", ["small", "cfg1", "cfg2"])
+def test_fused_launches_equal_timed_separate_launches(kind):
+    """The per-kernel timing mode launches every kernel on its own and in
+    order on one stream (the charge sites and chunk spheres apart from the
+    frame passes, force pass 2 without branches); the step graph fuses and
+    branches them.  Both give the same three steps bit for bit."""
+    print(f"case: fused against separate launches, {kind}", flush=True)
+    fib, st = Fb.make_config(2) if kind == "cfg2" else deck(kind)
+    g = O.Rng(9)
+    nD3 = 3 * (fib.n + fib.n_tail + fib.n_LH)
+    steps = [g.step_noise(s, fib.r_var(), nD3) for s in range(3)]
+    outs = []
+    for timed in (False, True):
+        with E.BDEngine(fib, use_graph=not timed) as eng:
+            eng.set_timing(timed)
+            eng.set_state(st, rebuild=True)
+            es = [eng.step(rr, p) for rr, p in steps]
+            outs.append((es, eng.get_state(), eng.get_derived()))
+    assert outs[0][0] == outs[1][0]
+    for k in ("r", "a", "b", "c", "r_t", "r_lh"):
+        assert np.array_equal(getattr(outs[0][1], k), getattr(outs[1][1], k)), k
+    for k, v in outs[0][2].items():
+        assert np.array_equal(v, outs[1][2][k]), k
+
+
 def test_device_noise_path_equals_host_noise_path():
     fib, st = deck("tiny")
     rr, p = noise(fib)
